@@ -1,0 +1,201 @@
+"""Benchmark of the Huffman-with-gap-arrays hot path on one MI355X.
+
+One step = one pass of the path over the 1 GiB Zipf(s=1.1) buffer (uint8, 256 ranks, seeded
+rank-to-byte permutation), already resident in HBM:
+    histogram (GPU) -> codebook (host, boundary package-merge) -> encode (GPU) -> decode (GPU)
+`value` is uncompressed GB/s of the whole step.  Per-stage GB/s and the roofline of the
+dominant kernel come from HIP events recorded on the stream the kernels run on.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config zipf_1GiB]
+Prints exactly one JSON line on stdout.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(d_dev, sample_bytes, repeats):
+    """The oracle (single-thread C restatement of the reference path) on the first
+    `sample_bytes` of the same buffer: histogram + codebook + encode + decode per repeat."""
+    from oracle import oracle
+
+    oracle.load()
+    sample = d_dev[:sample_bytes].cpu().numpy()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        h = oracle.histogram(sample)
+        cb = oracle.codebook(h)
+        sz = oracle.sizes(h, cb["length"])
+        pay, gap = oracle.encode(sample, cb, sz)
+        out = oracle.decode(pay, gap, sz["payload_words"], sz["gap_elements"], cb["order_symbol"],
+                            cb["order_length"], sample.size)
+        times.append(time.perf_counter() - t0)
+        if not (out == sample).all():
+            raise RuntimeError("oracle round trip failed")
+    t = statistics.median(times)
+    return dict(value=round(sample.size / t / 1e9, 4), unit="GB/s", cores=1, kind="port",
+                sample=f"first {sample.size >> 20} MiB of the same buffer; histogram+codebook+"
+                       f"encode+decode, single thread, median of {repeats}",
+                seconds=round(t, 3))
+
+
+def load_traffic(config, kernel):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (profiles/*.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{config}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="zipf_1GiB")
+    ap.add_argument("--cpu-sample-mib", type=int, default=256)
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py measures one GPU only: this build has no multi-GPU path "
+                 "(the encoder would shard by input range with a 256-bin all-reduce and a scalar "
+                 "exclusive scan; see DESIGN.md).  Run with --gpus 1.")
+
+    import torch
+
+    from huffman_coding_gap_arrays_amd import codec, synth
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n, seed, cdf, perm = synth.spec(args.config)
+    d_in = torch.empty(n, dtype=torch.uint8, device=dev)
+    codec.synth_fill(d_in, seed, cdf, perm)
+
+    # sizes/buffers from one untimed pass
+    hist = codec.histogram(d_in).cpu().numpy()
+    cb = codec.codebook_build(hist)
+    sizes = codec.encode_sizes(cb, hist)
+    W, G, GW = int(sizes.payload_words), int(sizes.gap_elements), int(sizes.gap_words)
+    payload = torch.empty(W + 4, dtype=torch.int32, device=dev)
+    gap = torch.empty(GW + 4, dtype=torch.int32, device=dev)
+    out = torch.empty(n + 16, dtype=torch.uint8, device=dev)
+    ws = codec.Workspace(dev)
+    ws.get(max(codec._lib().hgap_encode_workspace_bytes(n),
+               codec._lib().hgap_decode_workspace_bytes(G)))
+    C = 4 * (W + GW)  # compressed bytes (payload + gap)
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    def step(record):
+        e = [ev() for _ in range(5)] if record else None
+        if record:
+            e[0].record(stream)
+        h = codec.histogram(d_in, stream)
+        if record:
+            e[1].record(stream)
+        hb = h.cpu().numpy()  # D2H of 256 counts (synchronizes), as the reference does
+        book = codec.codebook_build(hb)
+        sz = codec.encode_sizes(book, hb)
+        table = codec.DeviceTable(book, device=dev)
+        if record:
+            e[2].record(stream)
+        enc = codec.encode(d_in, cb=book, sizes=sz, stream=stream, workspace=ws,
+                           payload=payload, gap=gap)
+        if record:
+            e[3].record(stream)
+        codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n,
+                     out=out, stream=stream, workspace=ws)
+        if record:
+            e[4].record(stream)
+        return e
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    recs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        recs.append(step(True))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_hist = statistics.mean(r[0].elapsed_time(r[1]) for r in recs) / 1e3
+    t_book = statistics.mean(r[1].elapsed_time(r[2]) for r in recs) / 1e3
+    t_enc = statistics.mean(r[2].elapsed_time(r[3]) for r in recs) / 1e3
+    t_dec = statistics.mean(r[3].elapsed_time(r[4]) for r in recs) / 1e3
+    ms_per_step = wall / args.steps * 1e3
+
+    ok = bool(torch.equal(out[:n], d_in))
+    if not ok:
+        sys.exit("decoded output differs from the input")
+
+    stages = {
+        "histogram": dict(seconds=t_hist, alg_bytes=n),
+        "encode": dict(seconds=t_enc, alg_bytes=n + C),
+        "decode": dict(seconds=t_dec, alg_bytes=C + n),
+    }
+    per_stage = {}
+    for k, v in stages.items():
+        gbs = v["alg_bytes"] / v["seconds"] / 1e9
+        per_stage[k] = dict(ms=round(v["seconds"] * 1e3, 4),
+                            uncompressed_GBps=round(n / v["seconds"] / 1e9, 2),
+                            alg_GBps=round(gbs, 2), roofline_frac=round(gbs / HBM_PEAK_GBS, 4))
+    per_stage["codebook_host"] = dict(ms=round(t_book * 1e3, 4))
+    dom = max(("encode", "decode"), key=lambda k: stages[k]["seconds"])
+    kern = {"encode": "enc_kernel", "decode": "dec_kernel"}[dom]
+    achieved = stages[dom]["alg_bytes"] / stages[dom]["seconds"] / 1e9
+    roofline = dict(bound="hbm", kernel=kern, achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=load_traffic(args.config, kern),
+                    alg_bytes_per_launch=stages[dom]["alg_bytes"])
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(d_in, min(n, args.cpu_sample_mib << 20), args.cpu_repeats)
+    value = n / (ms_per_step / 1e3) / 1e9
+    line = {
+        "metric": "encode/decode GB/s vs 8 TB/s HBM roofline (uncompressed bytes per second of "
+                  "histogram+codebook+encode+decode)",
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (counter-based splitmix64 generator on the GPU)",
+        "config": {"workload": f"{args.config}: {n} bytes uint8, "
+                               + ("Zipf s=1.1 over 256 ranks, seeded rank-to-byte permutation"
+                                  if args.config == "zipf_1GiB" else args.config),
+                   "n_bytes": n, "compressed_bytes": C, "payload_words": W, "gap_elements": G,
+                   "symbols": int(cb.symbol_count), "parallelism": "single GPU"},
+        "stages": per_stage,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "round_trip_exact": ok,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

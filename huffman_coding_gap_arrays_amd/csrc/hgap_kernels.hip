@@ -1,0 +1,736 @@
+// hgap_kernels.hip -- gfx950 kernels and launchers of the Huffman-with-gap-arrays path.
+//
+//   hist_kernel     byte histogram; one private 256-bin array per lane index in LDS, merged once
+//                   per workgroup into the global histogram
+//   enc_kernel      persistent encoder: per-thread code lengths, workgroup scan, single-pass
+//                   decoupled look-back across tiles (64-bit status words in HBM), bit-packing
+//                   into an LDS tile with gap nibbles emitted on the fly, 128-bit stores
+//   dec_kernel      persistent decoder: two-level table in LDS, one lane per 128-bit segment
+//                   starting at its gap offset, symbol-count scan + look-back, LDS compaction
+//                   and 128-bit stores
+//
+// Reference citations are file:line relative to the reference repository root.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "hgap.h"
+#include "hgap_internal.h"
+
+namespace hgap {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------------------------
+// wave64 helpers
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave: four row_shr steps build sums inside each
+// 16-lane row, row_bcast:15 / row_bcast:31 carry row totals into the rows above.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_or_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block-wide (256 threads = 4 waves) exclusive scan; returns the exclusive prefix and the
+// block total.  `s_wave` is 4 words of LDS.  Contains one barrier.
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t *s_wave,
+                                                   uint32_t &total) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_sum(x);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    const uint32_t w0 = s_wave[0], w1 = s_wave[1], w2 = s_wave[2], w3 = s_wave[3];
+    const uint32_t before = (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + (wave > 2 ? w2 : 0u);
+    total = w0 + w1 + w2 + w3;
+    return before + incl - x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Decoupled look-back (single pass) over per-tile 64-bit status words in HBM.
+// Status word: 0 = nothing published; kFlagA | aggregate; kFlagP | inclusive prefix.  Flag and
+// value travel in one 64-bit word, so relaxed agent-scope atomics are enough.  Tile ids come
+// from an atomic counter, so every predecessor is resident or finished (forward progress).
+// Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`.
+constexpr uint64_t kFlagA = 1ull << 62;
+constexpr uint64_t kFlagP = 2ull << 62;
+constexpr uint64_t kValueMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate) {
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) status_store(&status[0], kFlagP | aggregate);
+        return 0;
+    }
+    if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
+    uint64_t excl = 0;
+    int64_t window = (int64_t)tile - 1;
+    while (true) {
+        const int64_t idx = window - (int64_t)lane;
+        uint64_t v = idx >= 0 ? status_load(&status[idx]) : kFlagP;
+        while (__ballot(v == 0) != 0) {
+            __builtin_amdgcn_s_sleep(1);
+            if (v == 0) v = status_load(&status[idx]);
+        }
+        const uint64_t pmask = __ballot((v & kFlagP) != 0);
+        if (pmask != 0) {
+            const uint32_t first = (uint32_t)__builtin_ctzll(pmask);  // nearest inclusive prefix
+            excl += wave_sum_u64(lane <= first ? (v & kValueMask) : 0ull);
+            break;
+        }
+        excl += wave_sum_u64(v & kValueMask);
+        window -= 64;
+    }
+    if (lane == 0) status_store(&status[tile], kFlagP | (excl + aggregate));
+    return excl;
+}
+
+// ---------------------------------------------------------------------------------------------
+// zero fill (used for the histogram, the gap array and the look-back workspace)
+__global__ void fill_zero_kernel(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
+        if (i < na)
+            a[i] = 0u;
+        else
+            b[i - na] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Histogram (replaces cu_histgram + mergeHistogram, encoder/src/encoder.cu:33-116).
+// Every lane index owns one 256-bin array in LDS (shared by the 4 waves of the workgroup, so
+// updates are LDS atomics, but lanes of one wave never collide on an address).  Bin b of copy
+// c sits at word c*256 + (b ^ c): equal bytes in different lanes fall in different banks.
+constexpr int kHistCopies = 64;
+
+__global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__restrict__ in,
+                                                            uint64_t n,
+                                                            uint32_t *__restrict__ hist) {
+    __shared__ uint32_t s_bins[kHistCopies * 256];
+    for (int i = threadIdx.x; i < kHistCopies * 256; i += kHistThreads) s_bins[i] = 0u;
+    __syncthreads();
+    const uint32_t copy = lane_id();
+    uint32_t *mine = s_bins + copy * 256;
+    const uint64_t vecs = n >> 4;
+    const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
+    const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
+    uint64_t v = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x;
+    // 4 independent 16-byte loads in flight per thread
+    for (; v + 3 * stride < vecs; v += 4 * stride) {
+        u32x4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = __builtin_nontemporal_load(&in4[v + u * stride]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) ^ copy], 1u);
+        }
+    }
+    for (; v < vecs; v += stride) {
+        const u32x4 q = __builtin_nontemporal_load(&in4[v]);
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) ^ copy], 1u);
+    }
+    // tail bytes (n % 16) in the first workgroup
+    if (blockIdx.x == 0) {
+        const uint64_t t = (vecs << 4) + threadIdx.x;
+        if (t < n) atomicAdd(&mine[in[t] ^ copy], 1u);
+    }
+    __syncthreads();
+    // thread b owns bin b: sum of the 64 copies, then one global atomic per bin per workgroup
+    uint32_t sum = 0;
+    for (int c = 0; c < kHistCopies; c++) sum += s_bins[c * 256 + (threadIdx.x ^ c)];
+    if (sum) atomicAdd(&hist[threadIdx.x], sum);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Encoder (replaces cuencoder + cu_get_gaparray, encoder/src/encoder.cu:142-379).
+//
+// Tile = 256 threads x 32 consecutive input bytes.  Per tile:
+//   1. each thread looks up its 32 codes (code << 5 | length) in an LDS copy of the codebook
+//      and sums the lengths;
+//   2. workgroup scan of the bit counts, then decoupled look-back for the tile's global bit
+//      offset E;
+//   3. each thread packs its codes MSB-first into 32-bit words of an LDS tile buffer whose word
+//      0 is global word (E/32) rounded down to a multiple of 4; a thread's first and last words
+//      may be shared with its neighbours (LDS atomic OR), all others are plain LDS stores.
+//      When a completed word closes a 128-bit segment and the current code spills past it,
+//      the spill (< 16) is that segment's gap nibble (encoder.cu:307-312), OR-ed into an LDS
+//      copy of the gap words;
+//   4. the tile owns global words [E/32, (E+T)/32) (the last tile also the final partial word);
+//      its first word also holds the last E%32 bits of the previous tile, which are rebuilt
+//      here from the previous tile's last input bytes instead of waiting for that tile;
+//      owned words are written with 128-bit stores;
+//   5. gap words whose eight segment boundaries all lie inside the tile are stored; the (at
+//      most two) gap words shared with neighbouring tiles are OR-ed into the pre-zeroed array.
+struct EncTable {
+    uint32_t e[256];  // (code << 5) | length, by byte value
+};
+
+constexpr int kEncOutWords = kEncTileBytes * HGAP_MAX_CODE_LEN / 32 + 8;
+constexpr int kEncGapWords = kEncTileBytes * HGAP_MAX_CODE_LEN / 1024 + 3;
+
+__global__ __launch_bounds__(kEncThreads) void enc_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, uint32_t num_tiles, EncTable table,
+    uint32_t *__restrict__ payload, uint32_t payload_words, uint32_t *__restrict__ gap,
+    uint32_t gap_words, uint32_t *__restrict__ counter, uint64_t *__restrict__ status) {
+    constexpr int K = kEncBytesPerThread;
+    __shared__ uint32_t s_tab[256];
+    __shared__ __attribute__((aligned(16))) uint32_t s_out[kEncOutWords];
+    __shared__ uint32_t s_gap[kEncGapWords];
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile;
+
+    for (int i = threadIdx.x; i < 256; i += kEncThreads) s_tab[i] = table.e[i];
+    const uint32_t lane = lane_id();
+
+    while (true) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        for (int i = threadIdx.x; i < kEncOutWords; i += kEncThreads) s_out[i] = 0u;
+        for (int i = threadIdx.x; i < kEncGapWords; i += kEncThreads) s_gap[i] = 0u;
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= num_tiles) break;
+
+        // 1. codes and bit count
+        const uint64_t tile_start = (uint64_t)tile * kEncTileBytes;
+        const uint64_t mine = tile_start + (uint64_t)threadIdx.x * K;
+        const int nb = mine >= n ? 0 : (n - mine >= (uint64_t)K ? K : (int)(n - mine));
+        uint32_t e[K];
+        uint32_t bits = 0;
+        if (nb == K) {
+            const u32x4 *p = reinterpret_cast<const u32x4 *>(in + mine);
+#pragma unroll
+            for (int v = 0; v < K / 16; v++) {
+                const u32x4 q = __builtin_nontemporal_load(&p[v]);
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    e[16 * v + j] = s_tab[(q[j >> 2] >> (8 * (j & 3))) & 0xFFu];
+                    bits += e[16 * v + j] & 31u;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                e[j] = 0u;  // length 0: appends nothing
+                if (j < nb) e[j] = s_tab[in[mine + j]];
+                bits += e[j] & 31u;
+            }
+        }
+
+        // 2. tile scan + look-back
+        uint32_t tile_bits;
+        const uint32_t excl = block_excl_sum(bits, s_wave, tile_bits);
+        if (threadIdx.x < 64) {
+            const uint64_t E = look_back(status, tile, tile_bits);
+            if (threadIdx.x == 0) s_excl = E;
+        }
+        __syncthreads();
+        const uint64_t E = s_excl;
+        const uint64_t base_w = (E >> 5) & ~3ull;   // global word of s_out[0]
+        const uint64_t gap_base = E >> 10;          // global gap word of s_gap[0]
+
+        // 3. pack
+        if (nb > 0) {
+            const uint64_t B = E + excl;
+            uint32_t w = (uint32_t)((B >> 5) - base_w);
+            int nacc = (int)(B & 31);
+            uint64_t acc = 0;
+            bool first = true;
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                const uint32_t l = e[j] & 31u;
+                acc = (acc << l) | (e[j] >> 5);
+                nacc += (int)l;
+                if (nacc >= 32) {
+                    nacc -= 32;
+                    const uint32_t word = (uint32_t)(acc >> nacc);
+                    if (first)
+                        atomicOr(&s_out[w], word);
+                    else
+                        s_out[w] = word;
+                    first = false;
+                    if ((w & 3u) == 3u && nacc != 0) {
+                        const uint64_t seg = (base_w + w) >> 2;
+                        atomicOr(&s_gap[(seg >> 3) - gap_base], (uint32_t)nacc << ((seg & 7) * 4));
+                    }
+                    w++;
+                }
+            }
+            if (nacc > 0) atomicOr(&s_out[w], (uint32_t)(acc << (32 - nacc)));
+        }
+        __syncthreads();
+
+        // 4a. first word: add the previous tile's last E%32 bits
+        const uint32_t r = (uint32_t)(E & 31);
+        if (threadIdx.x < 64 && r != 0) {
+            uint32_t l = 0, c = 0;
+            if (lane < 32) {
+                const uint32_t ee = s_tab[in[tile_start - 1 - lane]];
+                l = ee & 31u;
+                c = ee >> 5;
+            }
+            const uint32_t cum = wave_incl_sum(l);
+            const uint32_t after = cum - l;  // bits between this code's end and the tile start
+            const uint32_t part = (lane < 32 && after < r) ? (c << (32 - (r - after))) : 0u;
+            const uint32_t prev_bits = wave_or_u32(part);
+            if (threadIdx.x == 0) s_out[(E >> 5) - base_w] |= prev_bits;
+        }
+        __syncthreads();
+
+        // 4b. payload words
+        const uint64_t T = tile_bits;
+        const uint64_t own_lo = E >> 5;
+        uint64_t own_hi = (tile + 1 == num_tiles) ? ((E + T + 31) >> 5) : ((E + T) >> 5);
+        if (own_hi > payload_words) own_hi = payload_words;
+        if (own_hi > own_lo) {
+            const uint32_t chunks = (uint32_t)((own_hi - base_w + 3) >> 2);
+            for (uint32_t c = threadIdx.x; c < chunks; c += kEncThreads) {
+                const uint64_t g = base_w + 4ull * c;
+                if (g >= own_lo && g + 4 <= own_hi) {
+                    const u32x4 v = *reinterpret_cast<const u32x4 *>(&s_out[4 * c]);
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(payload + g));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        if (g + i >= own_lo && g + i < own_hi) payload[g + i] = s_out[4 * c + i];
+                }
+            }
+        }
+
+        // 5. gap words
+        if (T > 0) {
+            const uint64_t g_hi = (E + T - 1) >> 10;
+            for (uint64_t g = gap_base + threadIdx.x; g <= g_hi; g += kEncThreads) {
+                if (g >= gap_words) break;
+                const uint32_t val = s_gap[g - gap_base];
+                const bool inside = (g * 1024 + 128 > E) && ((g + 1) * 1024 <= E + T);
+                if (inside)
+                    gap[g] = val;
+                else if (val)
+                    atomicOr(&gap[g], val);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Decoder (replaces gpu_dec_l1_l2, decoder/src/decoder.cu:454-730).
+//
+// Tile = 256 consecutive 128-bit segments, one per thread.  Segment s starts at bit
+// 128*s + (gap element s-1) and decodes every codeword that starts inside it (and inside the
+// payload), as in decoder.cu:505-569.  Per tile:
+//   1. each lane decodes its segment once with a 64-bit bit buffer and the two-level table in
+//      LDS, writing symbols into its own LDS slot (slots are an odd number of words apart so
+//      neighbouring lanes hit different banks);
+//   2. workgroup scan of the symbol counts, decoupled look-back for the tile's output offset O;
+//   3. lanes pull their slot into registers, then (after a barrier) write it compacted into the
+//      same LDS space at (O mod 16) + prefix, so the LDS image is aligned like the output;
+//   4. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
+//      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
+template <int NDW, bool SINGLE>
+__global__ __launch_bounds__(kDecThreads) void dec_kernel(
+    const uint32_t *__restrict__ payload, uint32_t payload_words,
+    const uint32_t *__restrict__ gap, uint32_t gap_elements, const uint32_t *__restrict__ table,
+    uint32_t table_words, uint32_t slot_offset, uint32_t prefix_bits, uint32_t l1_entries,
+    uint8_t *__restrict__ out,
+    uint64_t n, uint32_t num_tiles, uint32_t *__restrict__ counter,
+    uint64_t *__restrict__ status) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    __shared__ uint32_t s_wave[4];
+    __shared__ uint64_t s_base;
+    __shared__ uint32_t s_tile;
+    uint32_t *s_tab = s_dyn;
+    uint32_t *s_slot = s_dyn + slot_offset;  // 16-byte aligned; 256 * NDW words (+12)
+    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_slot);
+
+    for (uint32_t i = threadIdx.x; i < table_words; i += kDecThreads) s_tab[i] = table[i];
+    const uint32_t P = prefix_bits;
+    const uint32_t lane = lane_id();
+
+    while (true) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile >= num_tiles) break;
+
+        // 1. decode this lane's segment into its slot
+        const uint32_t seg = tile * kDecThreads + threadIdx.x;
+        uint32_t cnt = 0;
+        uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
+        if (seg < gap_elements) {
+            uint32_t start = 0;
+            if (seg > 0) start = (gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
+            const uint64_t w0i = 4ull * seg;
+            uint32_t w[5];
+            if (w0i + 5 <= payload_words) {
+                const u32x4 q = *reinterpret_cast<const u32x4 *>(payload + w0i);
+                w[0] = q[0]; w[1] = q[1]; w[2] = q[2]; w[3] = q[3];
+                w[4] = payload[w0i + 4];
+            } else {
+#pragma unroll
+                for (int i = 0; i < 5; i++) w[i] = (w0i + i < payload_words) ? payload[w0i + i] : 0u;
+            }
+            const uint64_t avail = 32ull * payload_words - 128ull * seg;
+            const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
+            uint64_t buf = ((((uint64_t)w[0]) << 32) | w[1]) << start;
+            int q = 64 - (int)start;
+            uint32_t n2 = w[2], n3 = w[3], n4 = w[4];
+            uint32_t p = start;
+            while (p < limit) {
+                const uint32_t hi = (uint32_t)(buf >> 32);
+                uint32_t ent = s_tab[hi >> (32 - P)];
+                if (!SINGLE && (ent >> 31)) {
+                    const uint32_t width = (ent >> 24) & 31u;
+                    const uint32_t idx2 = (hi << P) >> (32 - width);
+                    ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
+                }
+                const uint32_t l = (ent >> 8) & 31u;
+                slot[cnt++] = (uint8_t)ent;
+                p += l;
+                buf <<= l;
+                q -= (int)l;
+                if (q < 32) {
+                    buf |= ((uint64_t)n2) << (32 - q);
+                    q += 32;
+                    n2 = n3;
+                    n3 = n4;
+                    n4 = 0;
+                }
+            }
+        }
+
+        // 2. scan + look-back
+        uint32_t tile_syms;
+        const uint32_t excl = block_excl_sum(cnt, s_wave, tile_syms);
+        // pull the slot into registers while wave 0 looks back
+        uint32_t R[NDW];
+        const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
+#pragma unroll
+        for (int j = 0; j < NDW; j++) R[j] = (4u * j < cnt) ? slot_w[j] : 0u;
+        if (threadIdx.x < 64) {
+            const uint64_t O = look_back(status, tile, tile_syms);
+            if (threadIdx.x == 0) s_base = O;
+        }
+        __syncthreads();
+        const uint64_t O = s_base;
+
+        // 3. compact into LDS, aligned like the output
+        const uint32_t a = (uint32_t)(O & 15u) + excl;
+#pragma unroll
+        for (int j = 0; j < NDW; j++) {
+            if (__ballot(4u * j < cnt) == 0) break;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint32_t i = 4u * j + b;
+                if (i < cnt) s_bytes[a + i] = (uint8_t)(R[j] >> (8 * b));
+            }
+        }
+        __syncthreads();
+
+        // 4. store owned bytes
+        const uint64_t lo = O;
+        uint64_t hi = O + tile_syms;
+        if (hi > n) hi = n;
+        if (hi > lo) {
+            const uint64_t base = O & ~15ull;
+            const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
+            for (uint32_t c = threadIdx.x; c < chunks; c += kDecThreads) {
+                const uint64_t g = base + 16ull * c;
+                if (g >= lo && g + 16 <= hi) {
+                    const u32x4 v = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(out + g));
+                } else {
+                    for (int i = 0; i < 16; i++)
+                        if (g + i >= lo && g + i < hi) out[g + i] = s_bytes[16 * c + i];
+                }
+            }
+        }
+        (void)lane;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Synthetic data (counter-based splitmix64 + integer inverse CDF).
+struct SynthTable {
+    uint64_t cdf[256];
+    uint8_t perm[256];
+};
+
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void synth_kernel(uint8_t *__restrict__ out, uint64_t n,
+                                                    uint64_t start, uint64_t seed,
+                                                    SynthTable t) {
+    __shared__ uint64_t s_cdf[256];
+    __shared__ uint8_t s_perm[256];
+    s_cdf[threadIdx.x] = t.cdf[threadIdx.x];
+    s_perm[threadIdx.x] = t.perm[threadIdx.x];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 16;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16; i0 < n;
+         i0 += stride) {
+        uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const uint64_t r = splitmix64_at(seed, start + i0 + j) >> 11;
+            uint32_t lo = 0, hi = 255;
+#pragma unroll
+            for (int s = 0; s < 8; s++) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (r < s_cdf[mid]) hi = mid; else lo = mid + 1;
+            }
+            w[j >> 2] |= (uint32_t)s_perm[lo] << (8 * (j & 3));
+        }
+        if (i0 + 16 <= n) {
+            const u32x4 v = {w[0], w[1], w[2], w[3]};
+            *reinterpret_cast<u32x4 *>(out + i0) = v;
+        } else {
+            for (uint64_t j = 0; i0 + j < n; j++) out[i0 + j] = (uint8_t)(w[j >> 2] >> (8 * (j & 3)));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// launch helpers
+
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+static int device_cus() {
+    static int cached = -2;
+    if (cached == -2) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+            cached = -1;
+        else
+            cached = prop.multiProcessorCount;
+    }
+    return cached;
+}
+
+template <typename K>
+static int resident_blocks(K kernel, int threads, size_t dyn_lds) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, dyn_lds) !=
+            hipSuccess || per_cu < 1)
+        per_cu = 1;
+    const int cus = device_cus();
+    return per_cu * (cus > 0 ? cus : 1);
+}
+
+static hgap_status launch_zero(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb,
+                               hipStream_t s) {
+    if (na + nb == 0) return HGAP_OK;
+    uint64_t blocks = (na + nb + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, na, b, nb);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+template <int NDW, bool SINGLE>
+static hgap_status launch_dec(const uint32_t *payload, uint32_t W, const uint32_t *gap,
+                              uint32_t G, const uint32_t *table, const hgap_dectable_info *info,
+                              uint8_t *out, uint64_t n, uint32_t tiles, uint32_t *ws,
+                              hipStream_t s) {
+    const uint32_t table_words = info->l1_entries + info->l2_entries;
+    const uint32_t slot_offset = (table_words + 3u) & ~3u;
+    // staging region + 48 bytes: the last 16-byte copy-out chunk may read up to 29 bytes past
+    // 256 x NDW words (the bytes past the tile's output are never stored)
+    const size_t lds = 4ull * slot_offset + 4ull * (kDecThreads * NDW + 12);
+    if (lds > 160 * 1024 - 64) return HGAP_ERR_UNSUPPORTED;
+    auto kern = dec_kernel<NDW, SINGLE>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
+        attr_set = true;
+    }
+    const int grid = resident_blocks(kern, kDecThreads, lds);
+    const unsigned blocks = (unsigned)(tiles < (uint32_t)grid ? tiles : (uint32_t)grid);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDecThreads), lds, s, payload, W, gap, G, table,
+                       table_words, slot_offset, info->prefix_bits, info->l1_entries, out, n,
+                       tiles, ws,
+                       reinterpret_cast<uint64_t *>(ws + 4));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+}  // namespace hgap
+
+using namespace hgap;
+
+extern "C" {
+
+int hgap_device_cu_count(void) { return device_cus(); }
+
+hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
+                           hgap_stream_t stream) {
+    if (!d_hist || (n > 0 && !d_in)) return HGAP_ERR_INVALID_ARG;
+    if (!aligned16(d_in) || !aligned16(d_hist)) return HGAP_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hgap_status st = launch_zero(d_hist, 256, nullptr, 0, s);
+    if (st != HGAP_OK || n == 0) return st;
+    static int grid = 0;
+    if (grid == 0) grid = resident_blocks(hist_kernel, kHistThreads, 0);
+    uint64_t want = ((n >> 4) + kHistThreads * 4 - 1) / (kHistThreads * 4);
+    if (want < 1) want = 1;
+    const unsigned blocks = (unsigned)(want < (uint64_t)grid ? want : (uint64_t)grid);
+    hipLaunchKernelGGL(hist_kernel, dim3(blocks), dim3(kHistThreads), 0, s,
+                       static_cast<const uint8_t *>(d_in), n, d_hist);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+size_t hgap_encode_workspace_bytes(uint64_t n) {
+    const uint64_t tiles = (n + kEncTileBytes - 1) / kEncTileBytes;
+    return 16 + 8 * tiles;
+}
+
+hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                        const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+    if (!cb || !sizes) return HGAP_ERR_INVALID_ARG;
+    if (n == 0) return (sizes->payload_words == 0 && sizes->gap_elements == 0) ? HGAP_OK
+                                                                            : HGAP_ERR_SIZE;
+    if (!d_in || !d_payload || !d_gap || !d_workspace) return HGAP_ERR_INVALID_ARG;
+    if (!aligned16(d_in) || !aligned16(d_payload) || !aligned16(d_gap) ||
+        !aligned16(d_workspace))
+        return HGAP_ERR_INVALID_ARG;
+    if (n > 0xFFFFFFFFull) return HGAP_ERR_SIZE;
+    if (workspace_bytes < hgap_encode_workspace_bytes(n)) return HGAP_ERR_WORKSPACE;
+    const uint64_t tiles64 = (n + kEncTileBytes - 1) / kEncTileBytes;
+    const uint32_t tiles = (uint32_t)tiles64;
+    const uint32_t gap_words = (uint32_t)(((uint64_t)sizes->gap_elements + 7) / 8);
+    if ((uint64_t)sizes->payload_words * 32 < sizes->bits ||
+        (sizes->bits > 0 && (uint64_t)sizes->payload_words * 32 >= sizes->bits + 32))
+        return HGAP_ERR_SIZE;
+    EncTable t;
+    for (int b = 0; b < 256; b++) {
+        const uint32_t l = cb->length[b];
+        if (l > HGAP_MAX_CODE_LEN || (l > 0 && (cb->code[b] >> l) != 0)) return HGAP_ERR_CODEBOOK;
+        t.e[b] = (cb->code[b] << 5) | l;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
+    hgap_status st = launch_zero(d_gap, gap_words, ws, 4 + 2 * tiles64, s);
+    if (st != HGAP_OK) return st;
+    static int grid = 0;
+    if (grid == 0) grid = resident_blocks(enc_kernel, kEncThreads, 0);
+    const unsigned blocks = (unsigned)(tiles < (uint32_t)grid ? tiles : (uint32_t)grid);
+    hipLaunchKernelGGL(enc_kernel, dim3(blocks), dim3(kEncThreads), 0, s,
+                       static_cast<const uint8_t *>(d_in), n, tiles, t, d_payload,
+                       sizes->payload_words, d_gap, gap_words, ws,
+                       reinterpret_cast<uint64_t *>(ws + 4));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+size_t hgap_decode_workspace_bytes(uint32_t gap_elements) {
+    const uint64_t tiles = ((uint64_t)gap_elements + kDecThreads - 1) / kDecThreads;
+    return 16 + 8 * tiles;
+}
+
+hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const uint32_t *d_gap,
+                        uint32_t gap_elements, const void *d_table,
+                        const hgap_dectable_info *info, void *d_out, uint64_t n,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+    if (!info) return HGAP_ERR_INVALID_ARG;
+    if (n == 0) return HGAP_OK;
+    if (!d_payload || !d_gap || !d_table || !d_out || !d_workspace) return HGAP_ERR_INVALID_ARG;
+    if (!aligned16(d_payload) || !aligned16(d_gap) || !aligned16(d_table) || !aligned16(d_out) ||
+        !aligned16(d_workspace))
+        return HGAP_ERR_INVALID_ARG;
+    if ((uint64_t)gap_elements != ((uint64_t)payload_words + 3) / 4) return HGAP_ERR_SIZE;
+    if (info->symbol_count == 0 || info->prefix_bits < 1 || info->prefix_bits > 16 ||
+        info->l1_entries != (1u << info->prefix_bits) || info->min_length < 1)
+        return HGAP_ERR_CODEBOOK;
+    if (workspace_bytes < hgap_decode_workspace_bytes(gap_elements)) return HGAP_ERR_WORKSPACE;
+    const uint32_t tiles = (uint32_t)(((uint64_t)gap_elements + kDecThreads - 1) / kDecThreads);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
+    hgap_status st = launch_zero(ws, 4 + 2ull * tiles, nullptr, 0, s);
+    if (st != HGAP_OK) return st;
+    const uint32_t *tab = static_cast<const uint32_t *>(d_table);
+    uint8_t *out = static_cast<uint8_t *>(d_out);
+    const bool single = info->l2_entries == 0;
+    const uint32_t maxsym = info->max_symbols_per_segment;
+    // slot widths (words) are odd so that lanes' slots start in different LDS banks
+    if (maxsym <= 4 * 11) {
+        return single ? launch_dec<11, true>(d_payload, payload_words, d_gap, gap_elements, tab,
+                                             info, out, n, tiles, ws, s)
+                      : launch_dec<11, false>(d_payload, payload_words, d_gap, gap_elements,
+                                              tab, info, out, n, tiles, ws, s);
+    }
+    if (maxsym <= 4 * 17) {
+        return single ? launch_dec<17, true>(d_payload, payload_words, d_gap, gap_elements, tab,
+                                             info, out, n, tiles, ws, s)
+                      : launch_dec<17, false>(d_payload, payload_words, d_gap, gap_elements,
+                                              tab, info, out, n, tiles, ws, s);
+    }
+    return single ? launch_dec<33, true>(d_payload, payload_words, d_gap, gap_elements, tab, info,
+                                         out, n, tiles, ws, s)
+                  : launch_dec<33, false>(d_payload, payload_words, d_gap, gap_elements, tab,
+                                          info, out, n, tiles, ws, s);
+}
+
+hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,
+                            const uint64_t cdf[256], const uint8_t perm[256],
+                            hgap_stream_t stream) {
+    if (n == 0) return HGAP_OK;
+    if (!d_out || !cdf || !perm || !aligned16(d_out)) return HGAP_ERR_INVALID_ARG;
+    SynthTable t;
+    std::memcpy(t.cdf, cdf, sizeof(t.cdf));
+    std::memcpy(t.perm, perm, sizeof(t.perm));
+    uint64_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(synth_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), static_cast<uint8_t *>(d_out), n, start,
+                       seed, t);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+}  // extern "C"

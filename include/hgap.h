@@ -1,0 +1,149 @@
+/*
+ * hgap.h -- C ABI of the MI355X (gfx950) Huffman encoder/decoder with gap arrays.
+ *
+ * Stream format (bit-identical to the reference; reference paths relative to its root):
+ *   u64 k; k x {u8 symbol, u8 length} in code-assignment order (descending count, equal counts
+ *   in descending byte value); u32 n; u32 payload_words W; u32 gap_elements G;
+ *   u32 gap[ceil(G/8)]; u32 payload[W]                       (encoder/src/huff.cpp:111-125)
+ *   payload: codewords concatenated MSB-first in 32-bit words, zero tail;
+ *   gap element s: number of bits by which the codeword that starts in 128-bit segment s spills
+ *   into segment s+1 (0 when none does); 8 elements per u32, low nibble first
+ *   (encoder/src/encoder.cu:307-312,371-377).
+ *
+ * Device entry points are asynchronous on the given HIP stream (a hipStream_t passed as void*;
+ * NULL = the default stream), take caller-owned device buffers and an explicit workspace, and
+ * never allocate.  Every function returns an hgap_status; nothing calls exit().
+ * Device pointers (inputs, payload, gap, outputs, workspace, tables) must be 16-byte aligned.
+ */
+#ifndef HGAP_H
+#define HGAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *hgap_stream_t; /* hipStream_t */
+
+typedef enum hgap_status {
+    HGAP_OK = 0,
+    HGAP_ERR_INVALID_ARG = 1, /* null pointer, misaligned pointer, bad enum */
+    HGAP_ERR_SIZE = 2,        /* sizes disagree, or exceed the format's 32-bit fields */
+    HGAP_ERR_CODEBOOK = 3,    /* lengths > 16, not prefix-free, not monotone, symbol missing */
+    HGAP_ERR_WORKSPACE = 4,   /* workspace smaller than the *_workspace_bytes() figure */
+    HGAP_ERR_HIP = 5,         /* a HIP runtime call failed */
+    HGAP_ERR_FORMAT = 6,      /* stream truncated or header inconsistent */
+    HGAP_ERR_CAPACITY = 7,    /* caller buffer too small */
+    HGAP_ERR_UNSUPPORTED = 8  /* a configuration the kernels do not handle */
+} hgap_status;
+
+#define HGAP_MAX_CODE_LEN 16
+#define HGAP_SEGMENT_BITS 128
+#define HGAP_GAP_PER_WORD 8
+
+/* Codebook: the reference's Codetable[256] (encoder/include/constants.hpp:40-43) plus the
+ * code-assignment order the stream header lists. */
+typedef struct hgap_codebook {
+    uint32_t symbol_count;     /* k, 0..256 */
+    uint8_t order_symbol[256]; /* code-assignment (file) order */
+    uint8_t order_length[256];
+    uint32_t code[256];        /* by byte value; valid where length != 0 */
+    uint32_t length[256];      /* by byte value; 0 = byte absent */
+} hgap_codebook;
+
+/* Output sizes: encoder/src/symbols.cpp:45-55 and encoder/src/huff.cpp:84-88. */
+typedef struct hgap_sizes {
+    uint64_t bits;          /* sum of count x length */
+    uint32_t payload_words; /* ceil(bits / 32)  = W */
+    uint32_t gap_elements;  /* ceil(bits / 128) = G */
+    uint32_t gap_words;     /* ceil(G / 8) */
+    uint32_t reserved;
+} hgap_sizes;
+
+/* Decode table image (built on the host, copied to device memory by the caller).  Entry layout
+ * is internal to this library; only the sizes below are needed to place it. */
+typedef struct hgap_dectable_info {
+    uint32_t prefix_bits;   /* first-level index width */
+    uint32_t l1_entries;    /* 1 << prefix_bits */
+    uint32_t l2_entries;    /* second-level entries (0 when one level covers all codes) */
+    uint32_t bytes;         /* image size: 4 * (l1_entries + l2_entries) */
+    uint32_t min_length;    /* shortest code length */
+    uint32_t max_length;    /* longest code length */
+    uint32_t max_symbols_per_segment; /* bound on codewords starting in one 128-bit segment */
+    uint32_t symbol_count;
+} hgap_dectable_info;
+
+const char *hgap_status_string(int status);
+/* Device (ordinal) properties used by the launchers; -1 when no usable device. */
+int hgap_device_cu_count(void);
+
+/* ---- host-side steps of the path ---------------------------------------------------------- */
+
+/* store_symbols + qsort + boundary_PM + code assignment, bit-exact with
+ * encoder/src/symbols.cpp:29-43, encoder/src/huff.cpp:17-21,78-81 and
+ * encoder/src/package_merge.cpp:107-182.  A single distinct byte gets length 1, code 0; an
+ * all-zero histogram gives k = 0 (the reference cannot run on an empty input). */
+hgap_status hgap_codebook_build(const uint32_t hist[256], hgap_codebook *cb);
+
+/* Rebuilds the codebook from a stream header's (symbol, length) list using the canonical rule
+ * of decoder/src/get_table.cpp:17-20; checks lengths and prefix-freeness. */
+hgap_status hgap_codebook_from_order(const uint8_t *order_symbol, const uint8_t *order_length,
+                                     uint32_t k, hgap_codebook *cb);
+
+/* get_outputfilesize + size arithmetic (encoder/src/symbols.cpp:45-55, huff.cpp:84-88). */
+hgap_status hgap_encode_sizes(const hgap_codebook *cb, const uint32_t hist[256],
+                              hgap_sizes *sizes);
+
+/* Decode table; prefix_bits = 0 picks the default split.  `image` may be NULL to query info. */
+hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
+                                hgap_dectable_info *info, void *image, size_t image_cap);
+
+/* Stream header (encoder/src/huff.cpp:111-123; decoder/src/huff.cpp:27-39,71-74). */
+uint64_t hgap_stream_bytes(uint32_t k, uint32_t payload_words, uint32_t gap_elements);
+uint64_t hgap_stream_header_bytes(uint32_t k);
+hgap_status hgap_stream_write_header(const hgap_codebook *cb, uint32_t n,
+                                     const hgap_sizes *sizes, uint8_t *dst, size_t cap,
+                                     size_t *written);
+/* Parses a header; reports where the gap array and the payload start inside the stream. */
+hgap_status hgap_stream_parse_header(const uint8_t *src, size_t size, hgap_codebook *cb,
+                                     uint32_t *n, hgap_sizes *sizes, uint64_t *gap_offset,
+                                     uint64_t *payload_offset);
+
+/* ---- device steps (asynchronous on `stream`) ---------------------------------------------- */
+
+/* Byte histogram: d_hist[256] (u32) receives the counts of d_in[0..n).  Replaces
+ * histgram() (encoder/src/encoder.cu:118-140) without its per-call allocation. */
+hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
+                           hgap_stream_t stream);
+
+/* Encoder workspace: tile counter + one 64-bit look-back status word per input tile. */
+size_t hgap_encode_workspace_bytes(uint64_t n);
+/* Writes payload[W] and gap[ceil(G/8)] for d_in[0..n) with codebook `cb`; sizes must come from
+ * hgap_encode_sizes for the same input.  Replaces encode() (encoder/src/encoder.cu:382-457):
+ * no host copies, no memset of the payload, the caller keeps ownership of d_in. */
+hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                        const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
+
+/* Decoder workspace: tile counter + one 64-bit look-back status word per tile of segments. */
+size_t hgap_decode_workspace_bytes(uint32_t gap_elements);
+/* Decodes n bytes into d_out from payload[W] + gap array using a table image already in device
+ * memory (d_table, described by `info`).  Replaces decoder_l1_l2
+ * (decoder/src/decoder.cu:732-815) minus its host copies and timing loop. */
+hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const uint32_t *d_gap,
+                        uint32_t gap_elements, const void *d_table,
+                        const hgap_dectable_info *info, void *d_out, uint64_t n,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
+
+/* Synthetic data for tests and benchmarks: byte i = perm[rank], rank = first r with
+ * (splitmix64(seed + (start+i+1)*0x9E3779B97F4A7C15) >> 11) < cdf[r]  (cdf[255] = 2^53). */
+hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,
+                            const uint64_t cdf[256], const uint8_t perm[256],
+                            hgap_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HGAP_H */
