@@ -16,7 +16,7 @@ __all__ = ["synth", "HgapError", "load", "codec"]
 def __getattr__(name):
     # torch-backed device API, imported lazily so that CPU-only tools need not import torch
     if name == "codec":
-        from . import codec
+        import importlib
 
-        return codec
+        return importlib.import_module(__name__ + ".codec")
     raise AttributeError(name)

@@ -4,6 +4,7 @@
 // Reference citations are file:line relative to the reference repository root.
 #include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "hgap.h"
 #include "hgap_internal.h"
@@ -115,6 +116,10 @@ void package_merge_lengths(const uint64_t *w, int k, uint8_t *len) {
     delete pm;
 }
 
+static inline uint32_t single_entry(uint32_t sym, uint32_t len) {
+    return sym | (len << 8) | (len << 24);
+}
+
 static bool assign_codes(hgap_codebook *cb) {
     // package_merge.cpp:168-181 / decoder/src/get_table.cpp:17-20: code_0 = 0,
     // code_{t+1} = (code_t + 1) << (len_{t+1} - len_t), in code-assignment order.
@@ -222,11 +227,15 @@ hgap_status hgap_encode_sizes(const hgap_codebook *cb, const uint32_t hist[256],
     return HGAP_OK;
 }
 
-// Decode table image: first level indexed by the next `prefix_bits` stream bits; an entry is
-// either a codeword (symbol | length << 8) or, with bit 31 set, a pointer to a second-level
-// block (width << 24 | offset) indexed by the following `width` bits.  Unused first-level
-// entries (incomplete codes) decode as the first symbol with the longest length, so decoding
-// always advances by at least min_length bits.
+// Decode table image.  The first level is indexed by the next `prefix_bits` stream bits.  An
+// entry is either a pointer (bit 31 set: width << 24 | offset) to a second-level block indexed
+// by the following `width` bits, or a direct entry
+//     sym1 | total << 8 | two << 13 | sym2 << 16 | len1 << 24
+// where len1 is the first codeword's length and, when `two` is set, a second complete
+// codeword (sym2) also fits inside the indexed bits and `total` = len1 + len2 (else total =
+// len1).  Second-level entries are single codewords.  Unused entries (incomplete codes) decode
+// as the first symbol with the longest length, so decoding always advances by at least
+// min_length bits.
 hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
                                 hgap_dectable_info *info, void *image, size_t image_cap) {
     if (!cb || !info) return HGAP_ERR_INVALID_ARG;
@@ -281,7 +290,7 @@ hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
     if (!image) return HGAP_OK;
     if (image_cap < info->bytes) return HGAP_ERR_CAPACITY;
     uint32_t *tab = static_cast<uint32_t *>(image);
-    const uint32_t filler = (uint32_t)tmp.order_symbol[0] | (maxl << 8);
+    const uint32_t filler = single_entry(tmp.order_symbol[0], maxl);
     for (uint32_t i = 0; i < l1 + l2; i++) tab[i] = filler;
     uint32_t *sub = tab + l1;
     uint32_t next_off = 0;
@@ -291,7 +300,7 @@ hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
         const uint32_t l = tmp.order_length[t];
         const uint8_t s = tmp.order_symbol[t];
         const uint32_t c = tmp.code[s];
-        const uint32_t entry = (uint32_t)s | (l << 8);
+        const uint32_t entry = single_entry(s, l);
         if (l <= P) {
             const uint32_t first = c << (P - l), count = 1u << (P - l);
             for (uint32_t e = 0; e < count; e++) tab[first + e] = entry;
@@ -316,6 +325,24 @@ hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
         const uint32_t suffix = c & ((1u << sl) - 1u);
         const uint32_t first = suffix << (open_width - sl), count = 1u << (open_width - sl);
         for (uint32_t e = 0; e < count; e++) sub[open_off + first + e] = entry;
+    }
+    // pair up: a direct entry whose remaining P - len1 bits hold another complete codeword
+    // (a direct entry of length <= P - len1 at the left-aligned remainder) decodes both
+    if (P > minl) {
+        std::vector<uint32_t> one(tab, tab + l1);
+        for (uint32_t idx = 0; idx < l1; idx++) {
+            const uint32_t e1 = one[idx];
+            if (e1 >> 31) continue;
+            const uint32_t len1 = (e1 >> 24) & 31u;
+            if (len1 + minl > P) continue;
+            const uint32_t rest = (idx << len1) & (l1 - 1u);
+            const uint32_t e2 = one[rest];
+            if (e2 >> 31) continue;
+            const uint32_t len2 = (e2 >> 24) & 31u;
+            if (len1 + len2 > P) continue;
+            tab[idx] = (e1 & 0xFFu) | ((len1 + len2) << 8) | (1u << 13) | ((e2 & 0xFFu) << 16) |
+                       (len1 << 24);
+        }
     }
     return HGAP_OK;
 }

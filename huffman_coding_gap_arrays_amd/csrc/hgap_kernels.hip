@@ -189,9 +189,10 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
 //      and sums the lengths;
 //   2. workgroup scan of the bit counts, then decoupled look-back for the tile's global bit
 //      offset E;
-//   3. each thread packs its codes MSB-first into 32-bit words of an LDS tile buffer whose word
-//      0 is global word (E/32) rounded down to a multiple of 4; a thread's first and last words
-//      may be shared with its neighbours (LDS atomic OR), all others are plain LDS stores.
+//   3. each thread packs its codes (two at a time) MSB-first into 32-bit words of an LDS tile
+//      buffer whose word 0 is global word (E/32) rounded down to a multiple of 4; a thread's
+//      first and last words may be shared with its neighbours (LDS atomic OR), all others are
+//      plain LDS stores.
 //      When a completed word closes a 128-bit segment and the current code spills past it,
 //      the spill (< 16) is that segment's gap nibble (encoder.cu:307-312), OR-ed into an LDS
 //      copy of the gap words;
@@ -269,7 +270,8 @@ __global__ __launch_bounds__(kEncThreads) void enc_kernel(
         const uint64_t base_w = (E >> 5) & ~3ull;   // global word of s_out[0]
         const uint64_t gap_base = E >> 10;          // global gap word of s_gap[0]
 
-        // 3. pack
+        // 3. pack, two codes at a time: a pair is at most 32 bits, so appending it to at
+        // most 31 pending bits completes at most one word
         if (nb > 0) {
             const uint64_t B = E + excl;
             uint32_t w = (uint32_t)((B >> 5) - base_w);
@@ -277,10 +279,11 @@ __global__ __launch_bounds__(kEncThreads) void enc_kernel(
             uint64_t acc = 0;
             bool first = true;
 #pragma unroll
-            for (int j = 0; j < K; j++) {
-                const uint32_t l = e[j] & 31u;
-                acc = (acc << l) | (e[j] >> 5);
-                nacc += (int)l;
+            for (int j = 0; j < K; j += 2) {
+                const uint32_t l1 = e[j] & 31u, l2 = e[j + 1] & 31u;
+                const uint32_t pair = ((e[j] >> 5) << l2) | (e[j + 1] >> 5);
+                acc = (acc << (l1 + l2)) | pair;
+                nacc += (int)(l1 + l2);
                 if (nacc >= 32) {
                     nacc -= 32;
                     const uint32_t word = (uint32_t)(acc >> nacc);
@@ -290,8 +293,14 @@ __global__ __launch_bounds__(kEncThreads) void enc_kernel(
                         s_out[w] = word;
                     first = false;
                     if ((w & 3u) == 3u && nacc != 0) {
+                        // nacc bits lie past the segment boundary: all of them belong to the
+                        // second code if it straddles, else the first code straddles by
+                        // nacc - l2 (0 when the first code ends exactly on the boundary)
+                        const uint32_t spill = (uint32_t)nacc >= l2 ? (uint32_t)nacc - l2
+                                                                    : (uint32_t)nacc;
                         const uint64_t seg = (base_w + w) >> 2;
-                        atomicOr(&s_gap[(seg >> 3) - gap_base], (uint32_t)nacc << ((seg & 7) * 4));
+                        if (spill)
+                            atomicOr(&s_gap[(seg >> 3) - gap_base], spill << ((seg & 7) * 4));
                     }
                     w++;
                 }
@@ -417,6 +426,31 @@ __global__ __launch_bounds__(kDecThreads) void dec_kernel(
             int q = 64 - (int)start;
             uint32_t n2 = w[2], n3 = w[3], n4 = w[4];
             uint32_t p = start;
+            // entries may hold two codewords; both start before `limit` while p + P <= limit
+            while (p + P <= limit) {
+                const uint32_t hi = (uint32_t)(buf >> 32);
+                uint32_t ent = s_tab[hi >> (32 - P)];
+                if (!SINGLE && (ent >> 31)) {
+                    const uint32_t width = (ent >> 24) & 31u;
+                    const uint32_t idx2 = (hi << P) >> (32 - width);
+                    ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
+                }
+                const uint32_t t = (ent >> 8) & 31u;
+                slot[cnt] = (uint8_t)ent;
+                slot[cnt + 1] = (uint8_t)(ent >> 16);
+                cnt += 1u + ((ent >> 13) & 1u);
+                p += t;
+                buf <<= t;
+                q -= (int)t;
+                if (q < 32) {
+                    buf |= ((uint64_t)n2) << (32 - q);
+                    q += 32;
+                    n2 = n3;
+                    n3 = n4;
+                    n4 = 0;
+                }
+            }
+            // last few codewords of the segment, one at a time
             while (p < limit) {
                 const uint32_t hi = (uint32_t)(buf >> 32);
                 uint32_t ent = s_tab[hi >> (32 - P)];
@@ -425,7 +459,7 @@ __global__ __launch_bounds__(kDecThreads) void dec_kernel(
                     const uint32_t idx2 = (hi << P) >> (32 - width);
                     ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
                 }
-                const uint32_t l = (ent >> 8) & 31u;
+                const uint32_t l = (ent >> 24) & 31u;
                 slot[cnt++] = (uint8_t)ent;
                 p += l;
                 buf <<= l;
@@ -699,13 +733,14 @@ hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const
     const bool single = info->l2_entries == 0;
     const uint32_t maxsym = info->max_symbols_per_segment;
     // slot widths (words) are odd so that lanes' slots start in different LDS banks
-    if (maxsym <= 4 * 11) {
+    // a slot holds maxsym symbols plus one byte written past the last (second-symbol store)
+    if (maxsym + 1 <= 4 * 11) {
         return single ? launch_dec<11, true>(d_payload, payload_words, d_gap, gap_elements, tab,
                                              info, out, n, tiles, ws, s)
                       : launch_dec<11, false>(d_payload, payload_words, d_gap, gap_elements,
                                               tab, info, out, n, tiles, ws, s);
     }
-    if (maxsym <= 4 * 17) {
+    if (maxsym + 1 <= 4 * 17) {
         return single ? launch_dec<17, true>(d_payload, payload_words, d_gap, gap_elements, tab,
                                              info, out, n, tiles, ws, s)
                       : launch_dec<17, false>(d_payload, payload_words, d_gap, gap_elements,
