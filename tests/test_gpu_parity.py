@@ -118,16 +118,26 @@ def test_encode_bit_exact_and_decode(codec, orc, dist):
             pytest.fail(f"{dist} n={n}: {bad.size} bytes differ, first at {bad[0]}")
 
 
-@pytest.mark.parametrize("prefix", [1, 6, 9, 12, 13])
+@pytest.mark.parametrize("prefix", [1, 4, 6, 9, 12, 13])
 def test_decode_prefix_splits(codec, orc, prefix):
-    """Every first-level width decodes the same bytes (one- and two-level tables)."""
+    """Every first-level width decodes the same bytes (one- and two-level tables); a table too
+    large for LDS is refused with HGAP_ERR_UNSUPPORTED instead of being launched."""
     cdf, perm = dist_spec("fib")
     n = 300_001
     say("decode prefix", prefix)
     d_dev = device_data(codec, n, 5, cdf, perm)
     enc = codec.encode(d_dev)
     table = codec.DeviceTable(enc.codebook, prefix_bits=prefix)
-    assert (table.info.l2_entries == 0) == (prefix >= table.info.max_length)
+    info = table.info
+    assert (info.l2_entries == 0) == (prefix >= info.max_length)
+    ndw = 11 if info.max_symbols_per_segment + 1 <= 44 else (
+        17 if info.max_symbols_per_segment + 1 <= 68 else 33)
+    lds = 4 * ((info.l1_entries + info.l2_entries + 3) // 4 * 4) + 4 * (256 * ndw + 12)
+    if lds > 160 * 1024 - 64:
+        with pytest.raises(codec.HgapError) as err:
+            codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
+        assert err.value.status == 8
+        return
     out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
     assert bool((out == d_dev).all())
 
@@ -272,3 +282,34 @@ def test_reference_named_pipeline(cuda, capi, orc):
                            stage.ctypes.data_as(ctypes.c_void_p), size, P, k, info)
         assert (res[:n] == d).all()
         hip.lib.hipFree(d_hist)
+
+
+def test_cli_tools_round_trip(cuda, orc, tmp_path):
+    """hgap_encoder / hgap_decoder: the written file is the oracle's stream byte for byte and
+    decodes back to the input (reference usage: encoder in out; decoder in [out])."""
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    bindir = os.path.join(root, "huffman_coding_gap_arrays_amd", "bin")
+    cdf, perm = dist_spec("geometric")
+    d = orc.synth(2_000_003, 0, 17, cdf, perm)
+    src = tmp_path / "in.bin"
+    src.write_bytes(d.tobytes())
+    say("cli encode")
+    r = subprocess.run([os.path.join(bindir, "hgap_encoder"), str(src), str(tmp_path / "c.huff")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    blob = (tmp_path / "c.huff").read_bytes()
+    h = orc.histogram(d)
+    ocb = orc.codebook(h)
+    sz = orc.sizes(h, ocb["length"])
+    pay, gap = orc.encode(d, ocb, sz)
+    assert blob == orc.stream(ocb, d.size, pay, gap, sz["payload_words"], sz["gap_elements"])
+    say("cli decode")
+    r = subprocess.run([os.path.join(bindir, "hgap_decoder"), str(tmp_path / "c.huff")],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "decodedfile").read_bytes() == d.tobytes()
+    r = subprocess.run([os.path.join(bindir, "hgap_decoder"), str(tmp_path / "missing")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
