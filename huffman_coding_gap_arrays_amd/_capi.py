@@ -69,6 +69,35 @@ class DectableInfo(ctypes.Structure):
     ]
 
 
+class LaunchConfig(ctypes.Structure):
+    """hgap_launch_config: launch shape of the device steps."""
+
+    _fields_ = [
+        ("enc_threads", ctypes.c_uint32),
+        ("enc_bytes_per_thread", ctypes.c_uint32),
+        ("dec_threads", ctypes.c_uint32),
+        ("lookback_window", ctypes.c_uint32),
+        ("blocks_per_cu", ctypes.c_uint32),
+        ("prefix_bits", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        c = default_config()
+        for k, v in d.items():
+            setattr(c, k, int(v))
+        return c
+
+
+def default_config():
+    c = LaunchConfig()
+    load().hgap_default_config(ctypes.byref(c))
+    return c
+
+
 class Symbol(ctypes.Structure):
     """struct Symbol (encoder/include/constants.hpp:34-38)."""
 
@@ -101,6 +130,7 @@ _INT = ctypes.c_int
 PROTOTYPES = {
     "hgap_status_string": (ctypes.c_char_p, [_INT]),
     "hgap_device_cu_count": (_INT, []),
+    "hgap_default_config": (None, [ctypes.c_void_p]),
     "hgap_codebook_build": (_INT, [_P, ctypes.POINTER(Codebook)]),
     "hgap_codebook_from_order": (_INT, [_P, _P, _U32, ctypes.POINTER(Codebook)]),
     "hgap_encode_sizes": (_INT, [ctypes.POINTER(Codebook), _P, ctypes.POINTER(Sizes)]),
@@ -132,7 +162,17 @@ PROTOTYPES = {
         _INT,
         [_P, _U64, ctypes.POINTER(Codebook), ctypes.POINTER(Sizes), _P, _P, _P, _SZ, _P],
     ),
+    "hgap_encode_cfg": (
+        _INT,
+        [_P, _U64, ctypes.POINTER(Codebook), ctypes.POINTER(Sizes), _P, _P, _P, _SZ,
+         ctypes.c_void_p, _P],
+    ),
     "hgap_decode_workspace_bytes": (_SZ, [_U32]),
+    "hgap_decode_cfg": (
+        _INT,
+        [_P, _U32, _P, _U32, _P, ctypes.POINTER(DectableInfo), _P, _U64, _P, _SZ,
+         ctypes.c_void_p, _P],
+    ),
     "hgap_decode": (
         _INT,
         [_P, _U32, _P, _U32, _P, ctypes.POINTER(DectableInfo), _P, _U64, _P, _SZ, _P],

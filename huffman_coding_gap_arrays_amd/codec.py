@@ -23,13 +23,15 @@ import numpy as np
 import torch
 
 from . import _capi
-from ._capi import Codebook, DectableInfo, HgapError, Sizes, check
+from ._capi import Codebook, DectableInfo, HgapError, LaunchConfig, Sizes, check, default_config
 
 __all__ = [
     "Codebook",
     "Sizes",
     "DectableInfo",
     "HgapError",
+    "LaunchConfig",
+    "default_config",
     "Encoded",
     "DeviceTable",
     "histogram",
@@ -158,8 +160,16 @@ class Workspace:
         return self.buf
 
 
+def _cfg_ref(config):
+    if config is None:
+        return None
+    if isinstance(config, dict):
+        config = LaunchConfig.from_dict(config)
+    return ctypes.byref(config)
+
+
 def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, payload=None,
-           gap=None):
+           gap=None, config=None):
     """Encode a device uint8 buffer.  Without `cb`, builds the histogram and codebook first
     (device histogram, host codebook) like the reference's main (encoder/src/huff.cpp:69-99)."""
     d_in = _as_device_u8(d_in)
@@ -179,8 +189,9 @@ def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, pa
         gap = _words(sizes.gap_words, dev)
     wsb = _lib().hgap_encode_workspace_bytes(n)
     ws = (workspace or Workspace(dev)).get(wsb)
-    check(_lib().hgap_encode(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes), _ptr(payload),
-                             _ptr(gap), _ptr(ws), ws.numel(), _stream(stream)), "hgap_encode")
+    check(_lib().hgap_encode_cfg(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
+                                 _ptr(payload), _ptr(gap), _ptr(ws), ws.numel(), _cfg_ref(config),
+                                 _stream(stream)), "hgap_encode")
     return Encoded(n=n, codebook=cb, sizes=sizes, payload=payload, gap=gap)
 
 
@@ -203,16 +214,17 @@ class DeviceTable:
 
 
 def decode(payload, gap, payload_words, gap_elements, table, n, out=None, stream=None,
-           workspace=None):
+           workspace=None, config=None):
     """Decode n bytes from device payload/gap words with a DeviceTable; returns uint8 tensor."""
     dev = payload.device
     if out is None:
         out = torch.empty(max(int(n), 16), dtype=torch.uint8, device=dev)
     wsb = _lib().hgap_decode_workspace_bytes(gap_elements)
     ws = (workspace or Workspace(dev)).get(wsb)
-    check(_lib().hgap_decode(_ptr(payload), payload_words, _ptr(gap), gap_elements,
-                             _ptr(table.table), ctypes.byref(table.info), _ptr(out), n, _ptr(ws),
-                             ws.numel(), _stream(stream)), "hgap_decode")
+    check(_lib().hgap_decode_cfg(_ptr(payload), payload_words, _ptr(gap), gap_elements,
+                                 _ptr(table.table), ctypes.byref(table.info), _ptr(out), n,
+                                 _ptr(ws), ws.numel(), _cfg_ref(config), _stream(stream)),
+          "hgap_decode")
     return out[:n]
 
 

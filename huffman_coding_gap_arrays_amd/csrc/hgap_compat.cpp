@@ -308,7 +308,7 @@ void decoder_l1_l2(unsigned int *input, int inputfilesize, unsigned int *output,
     float t[3];
     for (int i = 0; i < 3; i++) COMPAT_HIP(hipEventElapsedTime(&t[i], ev[i], ev[i + 1]));
     printf("HtoD,%f, dec,%f, DtoH,%f", t[0], t[1], t[2]);
-    printf(",SEGMENTSIZE,%d,THREAD_NUM,%d\n", HGAP_SEGMENT_BITS, hgap::kDecThreads);
+    printf(",SEGMENTSIZE,%d,THREAD_NUM,%u\n", HGAP_SEGMENT_BITS, hgap::kDefaultDecThreads);
     for (auto &e : ev) COMPAT_HIP(hipEventDestroy(e));
     COMPAT_HIP(hipFree(d_payload));
     COMPAT_HIP(hipFree(d_gap));

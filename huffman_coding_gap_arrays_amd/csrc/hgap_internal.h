@@ -10,13 +10,17 @@ namespace hgap {
 // (8 KiB of LDS) cover every code of length <= 11 in a single lookup.
 constexpr uint32_t kDefaultPrefixBits = 11;
 
-// Encoder tiling: 256 threads x 32 input bytes = 8 KiB of input per tile.
-constexpr int kEncThreads = 256;
-constexpr int kEncBytesPerThread = 32;
-constexpr int kEncTileBytes = kEncThreads * kEncBytesPerThread;
-
-// Decoder tiling: 256 threads, one 128-bit segment per thread.
-constexpr int kDecThreads = 256;
+// Default launch shapes (override per call with hgap_launch_config).  Encoder tile = threads x
+// bytes per thread; decoder tile = threads segments of 128 bits.  Bigger tiles mean fewer
+// look-back participants; the look-back window is the number of predecessor status words read
+// per round.
+constexpr uint32_t kDefaultEncThreads = 1024;
+constexpr uint32_t kDefaultEncBytesPerThread = 32;
+constexpr uint32_t kDefaultDecThreads = 1024;
+constexpr uint32_t kDefaultLookbackWindow = 256;
+// smallest tiles any configuration uses (they size the look-back workspaces)
+constexpr uint32_t kMinEncTileBytes = 256 * 16;
+constexpr uint32_t kMinDecThreads = 256;
 
 // Histogram: 256 threads, one private bin array per lane index (64 copies of 256 bins).
 constexpr int kHistThreads = 256;

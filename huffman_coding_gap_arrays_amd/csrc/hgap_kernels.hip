@@ -59,18 +59,25 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// Block-wide (256 threads = 4 waves) exclusive scan; returns the exclusive prefix and the
-// block total.  `s_wave` is 4 words of LDS.  Contains one barrier.
+// Block-wide exclusive scan over THREADS threads (THREADS / 64 waves); returns the exclusive
+// prefix and the block total.  `s_wave` holds one word per wave.  Contains one barrier.
+template <int THREADS>
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t *s_wave,
                                                    uint32_t &total) {
+    constexpr int W = THREADS / 64;
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t incl = wave_incl_sum(x);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    const uint32_t w0 = s_wave[0], w1 = s_wave[1], w2 = s_wave[2], w3 = s_wave[3];
-    const uint32_t before = (wave > 0 ? w0 : 0u) + (wave > 1 ? w1 : 0u) + (wave > 2 ? w2 : 0u);
-    total = w0 + w1 + w2 + w3;
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const uint32_t v = s_wave[i];
+        tot += v;
+        before += (uint32_t)i < wave ? v : 0u;
+    }
+    total = tot;
     return before + incl - x;
 }
 
@@ -79,10 +86,15 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t x, uint32_t *s_wave,
 // Status word: 0 = nothing published; kFlagA | aggregate; kFlagP | inclusive prefix.  Flag and
 // value travel in one 64-bit word, so relaxed agent-scope atomics are enough.  Tile ids come
 // from an atomic counter, so every predecessor is resident or finished (forward progress).
+// Each round reads a window of 64 * wpl predecessors (wpl <= 4 per lane: lane l reads tiles
+// end - wpl*l - j, j < wpl) and stops at the nearest inclusive prefix.  A wide window lets
+// the prefix frontier keep up when many tiles are in flight: the tile rate a look-back can
+// sustain is about (window / round-trip latency).
 // Called by all 64 lanes of one wave; returns the exclusive prefix of `tile`.
 constexpr uint64_t kFlagA = 1ull << 62;
 constexpr uint64_t kFlagP = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 62) - 1;
+constexpr uint32_t kMaxWindowPerLane = 4;
 
 __device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -91,7 +103,8 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate) {
+__device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate,
+                              uint32_t wpl) {
     const uint32_t lane = lane_id();
     if (tile == 0) {
         if (lane == 0) status_store(&status[0], kFlagP | aggregate);
@@ -99,22 +112,42 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregat
     }
     if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
     uint64_t excl = 0;
-    int64_t window = (int64_t)tile - 1;
+    int64_t end = (int64_t)tile - 1;
     while (true) {
-        const int64_t idx = window - (int64_t)lane;
-        uint64_t v = idx >= 0 ? status_load(&status[idx]) : kFlagP;
-        while (__ballot(v == 0) != 0) {
-            __builtin_amdgcn_s_sleep(1);
-            if (v == 0) v = status_load(&status[idx]);
+        const int64_t first = end - (int64_t)(lane * wpl);
+        uint64_t v[kMaxWindowPerLane];
+#pragma unroll
+        for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
+            const int64_t idx = first - (int64_t)j;
+            // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
+            v[j] = j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP);
         }
-        const uint64_t pmask = __ballot((v & kFlagP) != 0);
+        while (true) {
+            bool missing = false;
+#pragma unroll
+            for (uint32_t j = 0; j < kMaxWindowPerLane; j++) missing |= v[j] == 0;
+            if (__ballot(missing) == 0) break;
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (uint32_t j = 0; j < kMaxWindowPerLane; j++)
+                if (v[j] == 0) v[j] = status_load(&status[first - (int64_t)j]);
+        }
+        // this lane's sum up to and including its nearest inclusive prefix (or all of it)
+        uint64_t sum = 0;
+        bool seen_p = false;
+#pragma unroll
+        for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
+            if (!seen_p) sum += v[j] & kValueMask;
+            seen_p |= (v[j] & kFlagP) != 0;
+        }
+        const uint64_t pmask = __ballot(seen_p);
         if (pmask != 0) {
-            const uint32_t first = (uint32_t)__builtin_ctzll(pmask);  // nearest inclusive prefix
-            excl += wave_sum_u64(lane <= first ? (v & kValueMask) : 0ull);
+            const uint32_t fl = (uint32_t)__builtin_ctzll(pmask);  // lane of nearest prefix
+            excl += wave_sum_u64(lane <= fl ? sum : 0ull);
             break;
         }
-        excl += wave_sum_u64(v & kValueMask);
-        window -= 64;
+        excl += wave_sum_u64(sum);
+        end -= 64 * (int64_t)wpl;
     }
     if (lane == 0) status_store(&status[tile], kFlagP | (excl + aggregate));
     return excl;
@@ -184,18 +217,17 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
 // ---------------------------------------------------------------------------------------------
 // Encoder (replaces cuencoder + cu_get_gaparray, encoder/src/encoder.cu:142-379).
 //
-// Tile = 256 threads x 32 consecutive input bytes.  Per tile:
-//   1. each thread looks up its 32 codes (code << 5 | length) in an LDS copy of the codebook
+// Tile = THREADS threads x K consecutive input bytes.  Per tile:
+//   1. each thread looks up its K codes (code << 5 | length) in an LDS copy of the codebook
 //      and sums the lengths;
 //   2. workgroup scan of the bit counts, then decoupled look-back for the tile's global bit
 //      offset E;
 //   3. each thread packs its codes (two at a time) MSB-first into 32-bit words of an LDS tile
 //      buffer whose word 0 is global word (E/32) rounded down to a multiple of 4; a thread's
 //      first and last words may be shared with its neighbours (LDS atomic OR), all others are
-//      plain LDS stores.
-//      When a completed word closes a 128-bit segment and the current code spills past it,
-//      the spill (< 16) is that segment's gap nibble (encoder.cu:307-312), OR-ed into an LDS
-//      copy of the gap words;
+//      plain LDS stores.  When a completed word closes a 128-bit segment and a code spills past
+//      it, the spill (< 16) is that segment's gap nibble (encoder.cu:307-312), OR-ed into an
+//      LDS copy of the gap words;
 //   4. the tile owns global words [E/32, (E+T)/32) (the last tile also the final partial word);
 //      its first word also holds the last E%32 bits of the previous tile, which are rebuilt
 //      here from the previous tile's last input bytes instead of waiting for that tile;
@@ -206,322 +238,376 @@ struct EncTable {
     uint32_t e[256];  // (code << 5) | length, by byte value
 };
 
-constexpr int kEncOutWords = kEncTileBytes * HGAP_MAX_CODE_LEN / 32 + 8;
-constexpr int kEncGapWords = kEncTileBytes * HGAP_MAX_CODE_LEN / 1024 + 3;
+struct EncBuffer {
+    const uint8_t *in;
+    uint64_t n;
+    uint32_t *payload;
+    uint32_t payload_words;
+    uint32_t gap_words;
+    uint32_t *gap;
+    uint64_t *status;  // one status word per tile of this buffer
+    uint32_t num_tiles;
+    uint32_t reserved;
+};
 
-__global__ __launch_bounds__(kEncThreads) void enc_kernel(
-    const uint8_t *__restrict__ in, uint64_t n, uint32_t num_tiles, EncTable table,
-    uint32_t *__restrict__ payload, uint32_t payload_words, uint32_t *__restrict__ gap,
-    uint32_t gap_words, uint32_t *__restrict__ counter, uint64_t *__restrict__ status) {
-    constexpr int K = kEncBytesPerThread;
+// LDS words of the output image for a tile of `tile_bytes` inputs whose codes are at most
+// `max_len` bits, and of its gap-word image
+__host__ __device__ constexpr uint32_t enc_out_words(uint32_t tile_bytes, uint32_t max_len) {
+    return (tile_bytes * max_len + 31) / 32 + 8;
+}
+__host__ __device__ constexpr uint32_t enc_gap_words(uint32_t tile_bytes, uint32_t max_len) {
+    return (tile_bytes * max_len) / 1024 + 3;
+}
+
+template <int THREADS, int K>
+__device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
+                                         const uint32_t *s_tab, uint32_t *s_out,
+                                         uint32_t *s_gap, uint32_t *s_wave, uint64_t *s_excl,
+                                         uint32_t wpl) {
+    constexpr uint32_t TILE = THREADS * K;
+    const uint32_t lane = lane_id();
+    const uint8_t *in = buf.in;
+    const uint64_t n = buf.n;
+
+    // 1. codes and bit count
+    const uint64_t tile_start = (uint64_t)tile * TILE;
+    const uint64_t mine = tile_start + (uint64_t)threadIdx.x * K;
+    const int nb = mine >= n ? 0 : (n - mine >= (uint64_t)K ? K : (int)(n - mine));
+    uint32_t e[K];
+    uint32_t bits = 0;
+    if (nb == K) {
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(in + mine);
+#pragma unroll
+        for (int v = 0; v < K / 16; v++) {
+            const u32x4 q = __builtin_nontemporal_load(&p[v]);
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                e[16 * v + j] = s_tab[(q[j >> 2] >> (8 * (j & 3))) & 0xFFu];
+                bits += e[16 * v + j] & 31u;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            e[j] = 0u;  // length 0: appends nothing
+            if (j < nb) e[j] = s_tab[in[mine + j]];
+            bits += e[j] & 31u;
+        }
+    }
+
+    // 2. tile scan, zero the used part of the LDS images, look-back
+    uint32_t tile_bits;
+    const uint32_t excl = block_excl_sum<THREADS>(bits, s_wave, tile_bits);
+    const uint32_t used_words = (tile_bits >> 5) + 8, used_gap = (tile_bits >> 10) + 3;
+    for (uint32_t i = threadIdx.x; i < used_words; i += THREADS) s_out[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < used_gap; i += THREADS) s_gap[i] = 0u;
+    if (threadIdx.x < 64) {
+        const uint64_t E = look_back(buf.status, tile, tile_bits, wpl);
+        if (threadIdx.x == 0) *s_excl = E;
+    }
+    __syncthreads();
+    const uint64_t E = *s_excl;
+    const uint64_t base_w = (E >> 5) & ~3ull;  // global word of s_out[0]
+    const uint64_t gap_base = E >> 10;         // global gap word of s_gap[0]
+
+    // 3. pack, two codes at a time: a pair is at most 32 bits, so appending it to at most 31
+    // pending bits completes at most one word
+    if (nb > 0) {
+        const uint64_t B = E + excl;
+        uint32_t w = (uint32_t)((B >> 5) - base_w);
+        int nacc = (int)(B & 31);
+        uint64_t acc = 0;
+        bool first = true;
+#pragma unroll
+        for (int j = 0; j < K; j += 2) {
+            const uint32_t l1 = e[j] & 31u, l2 = e[j + 1] & 31u;
+            const uint32_t pair = ((e[j] >> 5) << l2) | (e[j + 1] >> 5);
+            acc = (acc << (l1 + l2)) | pair;
+            nacc += (int)(l1 + l2);
+            if (nacc >= 32) {
+                nacc -= 32;
+                const uint32_t word = (uint32_t)(acc >> nacc);
+                if (first)
+                    atomicOr(&s_out[w], word);
+                else
+                    s_out[w] = word;
+                first = false;
+                if ((w & 3u) == 3u && nacc != 0) {
+                    // nacc bits lie past the segment boundary: all of them belong to the second
+                    // code if it straddles, else the first code straddles by nacc - l2 (0 when
+                    // the first code ends exactly on the boundary)
+                    const uint32_t spill =
+                        (uint32_t)nacc >= l2 ? (uint32_t)nacc - l2 : (uint32_t)nacc;
+                    const uint64_t seg = (base_w + w) >> 2;
+                    if (spill) atomicOr(&s_gap[(seg >> 3) - gap_base], spill << ((seg & 7) * 4));
+                }
+                w++;
+            }
+        }
+        if (nacc > 0) atomicOr(&s_out[w], (uint32_t)(acc << (32 - nacc)));
+    }
+    __syncthreads();
+
+    // 4a. first word: add the previous tile's last E%32 bits
+    const uint32_t r = (uint32_t)(E & 31);
+    if (threadIdx.x < 64 && r != 0) {
+        uint32_t l = 0, c = 0;
+        if (lane < 32) {
+            const uint32_t ee = s_tab[in[tile_start - 1 - lane]];
+            l = ee & 31u;
+            c = ee >> 5;
+        }
+        const uint32_t cum = wave_incl_sum(l);
+        const uint32_t after = cum - l;  // bits between this code's end and the tile start
+        const uint32_t part = (lane < 32 && after < r) ? (c << (32 - (r - after))) : 0u;
+        const uint32_t prev_bits = wave_or_u32(part);
+        if (threadIdx.x == 0) s_out[(E >> 5) - base_w] |= prev_bits;
+    }
+    __syncthreads();
+
+    // 4b. payload words
+    const uint64_t T = tile_bits;
+    const uint64_t own_lo = E >> 5;
+    uint64_t own_hi = (tile + 1 == buf.num_tiles) ? ((E + T + 31) >> 5) : ((E + T) >> 5);
+    if (own_hi > buf.payload_words) own_hi = buf.payload_words;
+    if (own_hi > own_lo) {
+        const uint32_t chunks = (uint32_t)((own_hi - base_w + 3) >> 2);
+        for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+            const uint64_t g = base_w + 4ull * c;
+            if (g >= own_lo && g + 4 <= own_hi) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(&s_out[4 * c]);
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.payload + g));
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (g + i >= own_lo && g + i < own_hi) buf.payload[g + i] = s_out[4 * c + i];
+            }
+        }
+    }
+
+    // 5. gap words
+    if (T > 0) {
+        const uint64_t g_hi = (E + T - 1) >> 10;
+        for (uint64_t g = gap_base + threadIdx.x; g <= g_hi; g += THREADS) {
+            if (g >= buf.gap_words) break;
+            const uint32_t val = s_gap[g - gap_base];
+            const bool inside = (g * 1024 + 128 > E) && ((g + 1) * 1024 <= E + T);
+            if (inside)
+                buf.gap[g] = val;
+            else if (val)
+                atomicOr(&buf.gap[g], val);
+        }
+    }
+    __syncthreads();
+}
+
+template <int THREADS, int K>
+__global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable table,
+                                                      uint32_t *__restrict__ counter,
+                                                      uint32_t out_words, uint32_t wpl) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_tab[256];
-    __shared__ __attribute__((aligned(16))) uint32_t s_out[kEncOutWords];
-    __shared__ uint32_t s_gap[kEncGapWords];
-    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_wave[THREADS / 64];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_tile;
-
-    for (int i = threadIdx.x; i < 256; i += kEncThreads) s_tab[i] = table.e[i];
-    const uint32_t lane = lane_id();
-
+    uint32_t *s_out = s_dyn;
+    uint32_t *s_gap = s_dyn + out_words;
+    for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = table.e[i];
     while (true) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        for (int i = threadIdx.x; i < kEncOutWords; i += kEncThreads) s_out[i] = 0u;
-        for (int i = threadIdx.x; i < kEncGapWords; i += kEncThreads) s_gap[i] = 0u;
         __syncthreads();
         const uint32_t tile = s_tile;
-        if (tile >= num_tiles) break;
-
-        // 1. codes and bit count
-        const uint64_t tile_start = (uint64_t)tile * kEncTileBytes;
-        const uint64_t mine = tile_start + (uint64_t)threadIdx.x * K;
-        const int nb = mine >= n ? 0 : (n - mine >= (uint64_t)K ? K : (int)(n - mine));
-        uint32_t e[K];
-        uint32_t bits = 0;
-        if (nb == K) {
-            const u32x4 *p = reinterpret_cast<const u32x4 *>(in + mine);
-#pragma unroll
-            for (int v = 0; v < K / 16; v++) {
-                const u32x4 q = __builtin_nontemporal_load(&p[v]);
-#pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    e[16 * v + j] = s_tab[(q[j >> 2] >> (8 * (j & 3))) & 0xFFu];
-                    bits += e[16 * v + j] & 31u;
-                }
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                e[j] = 0u;  // length 0: appends nothing
-                if (j < nb) e[j] = s_tab[in[mine + j]];
-                bits += e[j] & 31u;
-            }
-        }
-
-        // 2. tile scan + look-back
-        uint32_t tile_bits;
-        const uint32_t excl = block_excl_sum(bits, s_wave, tile_bits);
-        if (threadIdx.x < 64) {
-            const uint64_t E = look_back(status, tile, tile_bits);
-            if (threadIdx.x == 0) s_excl = E;
-        }
-        __syncthreads();
-        const uint64_t E = s_excl;
-        const uint64_t base_w = (E >> 5) & ~3ull;   // global word of s_out[0]
-        const uint64_t gap_base = E >> 10;          // global gap word of s_gap[0]
-
-        // 3. pack, two codes at a time: a pair is at most 32 bits, so appending it to at
-        // most 31 pending bits completes at most one word
-        if (nb > 0) {
-            const uint64_t B = E + excl;
-            uint32_t w = (uint32_t)((B >> 5) - base_w);
-            int nacc = (int)(B & 31);
-            uint64_t acc = 0;
-            bool first = true;
-#pragma unroll
-            for (int j = 0; j < K; j += 2) {
-                const uint32_t l1 = e[j] & 31u, l2 = e[j + 1] & 31u;
-                const uint32_t pair = ((e[j] >> 5) << l2) | (e[j + 1] >> 5);
-                acc = (acc << (l1 + l2)) | pair;
-                nacc += (int)(l1 + l2);
-                if (nacc >= 32) {
-                    nacc -= 32;
-                    const uint32_t word = (uint32_t)(acc >> nacc);
-                    if (first)
-                        atomicOr(&s_out[w], word);
-                    else
-                        s_out[w] = word;
-                    first = false;
-                    if ((w & 3u) == 3u && nacc != 0) {
-                        // nacc bits lie past the segment boundary: all of them belong to the
-                        // second code if it straddles, else the first code straddles by
-                        // nacc - l2 (0 when the first code ends exactly on the boundary)
-                        const uint32_t spill = (uint32_t)nacc >= l2 ? (uint32_t)nacc - l2
-                                                                    : (uint32_t)nacc;
-                        const uint64_t seg = (base_w + w) >> 2;
-                        if (spill)
-                            atomicOr(&s_gap[(seg >> 3) - gap_base], spill << ((seg & 7) * 4));
-                    }
-                    w++;
-                }
-            }
-            if (nacc > 0) atomicOr(&s_out[w], (uint32_t)(acc << (32 - nacc)));
-        }
-        __syncthreads();
-
-        // 4a. first word: add the previous tile's last E%32 bits
-        const uint32_t r = (uint32_t)(E & 31);
-        if (threadIdx.x < 64 && r != 0) {
-            uint32_t l = 0, c = 0;
-            if (lane < 32) {
-                const uint32_t ee = s_tab[in[tile_start - 1 - lane]];
-                l = ee & 31u;
-                c = ee >> 5;
-            }
-            const uint32_t cum = wave_incl_sum(l);
-            const uint32_t after = cum - l;  // bits between this code's end and the tile start
-            const uint32_t part = (lane < 32 && after < r) ? (c << (32 - (r - after))) : 0u;
-            const uint32_t prev_bits = wave_or_u32(part);
-            if (threadIdx.x == 0) s_out[(E >> 5) - base_w] |= prev_bits;
-        }
-        __syncthreads();
-
-        // 4b. payload words
-        const uint64_t T = tile_bits;
-        const uint64_t own_lo = E >> 5;
-        uint64_t own_hi = (tile + 1 == num_tiles) ? ((E + T + 31) >> 5) : ((E + T) >> 5);
-        if (own_hi > payload_words) own_hi = payload_words;
-        if (own_hi > own_lo) {
-            const uint32_t chunks = (uint32_t)((own_hi - base_w + 3) >> 2);
-            for (uint32_t c = threadIdx.x; c < chunks; c += kEncThreads) {
-                const uint64_t g = base_w + 4ull * c;
-                if (g >= own_lo && g + 4 <= own_hi) {
-                    const u32x4 v = *reinterpret_cast<const u32x4 *>(&s_out[4 * c]);
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(payload + g));
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; i++)
-                        if (g + i >= own_lo && g + i < own_hi) payload[g + i] = s_out[4 * c + i];
-                }
-            }
-        }
-
-        // 5. gap words
-        if (T > 0) {
-            const uint64_t g_hi = (E + T - 1) >> 10;
-            for (uint64_t g = gap_base + threadIdx.x; g <= g_hi; g += kEncThreads) {
-                if (g >= gap_words) break;
-                const uint32_t val = s_gap[g - gap_base];
-                const bool inside = (g * 1024 + 128 > E) && ((g + 1) * 1024 <= E + T);
-                if (inside)
-                    gap[g] = val;
-                else if (val)
-                    atomicOr(&gap[g], val);
-            }
-        }
-        __syncthreads();
+        if (tile >= buf.num_tiles) break;
+        enc_tile<THREADS, K>(buf, tile, s_tab, s_out, s_gap, s_wave, &s_excl, wpl);
     }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Decoder (replaces gpu_dec_l1_l2, decoder/src/decoder.cu:454-730).
 //
-// Tile = 256 consecutive 128-bit segments, one per thread.  Segment s starts at bit
+// Tile = THREADS consecutive 128-bit segments, one per thread.  Segment s starts at bit
 // 128*s + (gap element s-1) and decodes every codeword that starts inside it (and inside the
 // payload), as in decoder.cu:505-569.  Per tile:
 //   1. each lane decodes its segment once with a 64-bit bit buffer and the two-level table in
 //      LDS, writing symbols into its own LDS slot (slots are an odd number of words apart so
-//      neighbouring lanes hit different banks);
+//      neighbouring lanes hit different banks); first-level entries may yield two codewords;
 //   2. workgroup scan of the symbol counts, decoupled look-back for the tile's output offset O;
 //   3. lanes pull their slot into registers, then (after a barrier) write it compacted into the
 //      same LDS space at (O mod 16) + prefix, so the LDS image is aligned like the output;
 //   4. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
 //      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
-template <int NDW, bool SINGLE>
-__global__ __launch_bounds__(kDecThreads) void dec_kernel(
-    const uint32_t *__restrict__ payload, uint32_t payload_words,
-    const uint32_t *__restrict__ gap, uint32_t gap_elements, const uint32_t *__restrict__ table,
-    uint32_t table_words, uint32_t slot_offset, uint32_t prefix_bits, uint32_t l1_entries,
-    uint8_t *__restrict__ out,
-    uint64_t n, uint32_t num_tiles, uint32_t *__restrict__ counter,
-    uint64_t *__restrict__ status) {
+struct DecBuffer {
+    const uint32_t *payload;
+    uint32_t payload_words;
+    uint32_t gap_elements;
+    const uint32_t *gap;
+    uint8_t *out;
+    uint64_t n;
+    uint64_t *status;  // one status word per tile of this buffer
+    uint32_t num_tiles;
+    uint32_t reserved;
+};
+
+struct DecTable {
+    uint32_t prefix_bits;
+    uint32_t l1_entries;
+};
+
+template <bool SINGLE>
+__device__ __forceinline__ uint32_t dec_lookup(const uint32_t *s_tab, uint32_t hi, uint32_t P,
+                                               uint32_t l1_entries) {
+    uint32_t ent = s_tab[hi >> (32 - P)];
+    if (!SINGLE && (ent >> 31)) {
+        const uint32_t width = (ent >> 24) & 31u;
+        const uint32_t idx2 = (hi << P) >> (32 - width);
+        ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
+    }
+    return ent;
+}
+
+template <int THREADS, int NDW, bool SINGLE>
+__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, DecTable t,
+                                         const uint32_t *s_tab, uint8_t *s_bytes,
+                                         uint32_t *s_wave, uint64_t *s_base, uint32_t wpl) {
+    const uint32_t P = t.prefix_bits;
+    // 1. decode this lane's segment into its slot
+    const uint32_t seg = tile * THREADS + threadIdx.x;
+    uint32_t cnt = 0;
+    uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
+    if (seg < buf.gap_elements) {
+        uint32_t start = 0;
+        if (seg > 0) start = (buf.gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
+        const uint64_t w0i = 4ull * seg;
+        uint32_t w[5];
+        if (w0i + 5 <= buf.payload_words) {
+            const u32x4 q = *reinterpret_cast<const u32x4 *>(buf.payload + w0i);
+            w[0] = q[0];
+            w[1] = q[1];
+            w[2] = q[2];
+            w[3] = q[3];
+            w[4] = buf.payload[w0i + 4];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 5; i++)
+                w[i] = (w0i + i < buf.payload_words) ? buf.payload[w0i + i] : 0u;
+        }
+        const uint64_t avail = 32ull * buf.payload_words - 128ull * seg;
+        const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
+        uint64_t bb = ((((uint64_t)w[0]) << 32) | w[1]) << start;
+        int q = 64 - (int)start;
+        uint32_t n2 = w[2], n3 = w[3], n4 = w[4];
+        uint32_t p = start;
+        // entries may hold two codewords; both start before `limit` while p + P <= limit
+        while (p + P <= limit) {
+            const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
+            const uint32_t adv = (ent >> 8) & 31u;
+            slot[cnt] = (uint8_t)ent;
+            slot[cnt + 1] = (uint8_t)(ent >> 16);
+            cnt += 1u + ((ent >> 13) & 1u);
+            p += adv;
+            bb <<= adv;
+            q -= (int)adv;
+            if (q < 32) {
+                bb |= ((uint64_t)n2) << (32 - q);
+                q += 32;
+                n2 = n3;
+                n3 = n4;
+                n4 = 0;
+            }
+        }
+        // last few codewords of the segment, one at a time
+        while (p < limit) {
+            const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
+            const uint32_t adv = (ent >> 24) & 31u;
+            slot[cnt++] = (uint8_t)ent;
+            p += adv;
+            bb <<= adv;
+            q -= (int)adv;
+            if (q < 32) {
+                bb |= ((uint64_t)n2) << (32 - q);
+                q += 32;
+                n2 = n3;
+                n3 = n4;
+                n4 = 0;
+            }
+        }
+    }
+
+    // 2. scan + look-back
+    uint32_t tile_syms;
+    const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
+    // pull the slot into registers while wave 0 looks back
+    uint32_t R[NDW];
+    const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
+#pragma unroll
+    for (int j = 0; j < NDW; j++) R[j] = (4u * j < cnt) ? slot_w[j] : 0u;
+    if (threadIdx.x < 64) {
+        const uint64_t O = look_back(buf.status, tile, tile_syms, wpl);
+        if (threadIdx.x == 0) *s_base = O;
+    }
+    __syncthreads();
+    const uint64_t O = *s_base;
+
+    // 3. compact into LDS, aligned like the output
+    const uint32_t a = (uint32_t)(O & 15u) + excl;
+#pragma unroll
+    for (int j = 0; j < NDW; j++) {
+        if (__ballot(4u * j < cnt) == 0) break;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const uint32_t i = 4u * j + b;
+            if (i < cnt) s_bytes[a + i] = (uint8_t)(R[j] >> (8 * b));
+        }
+    }
+    __syncthreads();
+
+    // 4. store owned bytes
+    const uint64_t lo = O;
+    uint64_t hi = O + tile_syms;
+    if (hi > buf.n) hi = buf.n;
+    if (hi > lo) {
+        const uint64_t base = O & ~15ull;
+        const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
+        for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+            const uint64_t g = base + 16ull * c;
+            if (g >= lo && g + 16 <= hi) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + g));
+            } else {
+                for (int i = 0; i < 16; i++)
+                    if (g + i >= lo && g + i < hi) buf.out[g + i] = s_bytes[16 * c + i];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// LDS bytes of the decode tile staging region: THREADS slots of NDW words, plus 48 bytes
+// because the last 16-byte copy-out chunk may read up to 29 bytes past THREADS x NDW words
+// (bytes past the tile's output are never stored)
+__host__ __device__ constexpr uint32_t dec_stage_bytes(uint32_t threads, uint32_t ndw) {
+    return 4u * (threads * ndw + 12u);
+}
+
+template <int THREADS, int NDW, bool SINGLE>
+__global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
+                                                      const uint32_t *__restrict__ table,
+                                                      uint32_t table_words, uint32_t slot_offset,
+                                                      uint32_t *__restrict__ counter,
+                                                      uint32_t wpl) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
-    __shared__ uint32_t s_wave[4];
+    __shared__ uint32_t s_wave[THREADS / 64];
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_tile;
     uint32_t *s_tab = s_dyn;
-    uint32_t *s_slot = s_dyn + slot_offset;  // 16-byte aligned; 256 * NDW words (+12)
-    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_slot);
-
-    for (uint32_t i = threadIdx.x; i < table_words; i += kDecThreads) s_tab[i] = table[i];
-    const uint32_t P = prefix_bits;
-    const uint32_t lane = lane_id();
-
+    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_dyn + slot_offset);  // 16-byte aligned
+    for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
     while (true) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
-        if (tile >= num_tiles) break;
-
-        // 1. decode this lane's segment into its slot
-        const uint32_t seg = tile * kDecThreads + threadIdx.x;
-        uint32_t cnt = 0;
-        uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
-        if (seg < gap_elements) {
-            uint32_t start = 0;
-            if (seg > 0) start = (gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
-            const uint64_t w0i = 4ull * seg;
-            uint32_t w[5];
-            if (w0i + 5 <= payload_words) {
-                const u32x4 q = *reinterpret_cast<const u32x4 *>(payload + w0i);
-                w[0] = q[0]; w[1] = q[1]; w[2] = q[2]; w[3] = q[3];
-                w[4] = payload[w0i + 4];
-            } else {
-#pragma unroll
-                for (int i = 0; i < 5; i++) w[i] = (w0i + i < payload_words) ? payload[w0i + i] : 0u;
-            }
-            const uint64_t avail = 32ull * payload_words - 128ull * seg;
-            const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
-            uint64_t buf = ((((uint64_t)w[0]) << 32) | w[1]) << start;
-            int q = 64 - (int)start;
-            uint32_t n2 = w[2], n3 = w[3], n4 = w[4];
-            uint32_t p = start;
-            // entries may hold two codewords; both start before `limit` while p + P <= limit
-            while (p + P <= limit) {
-                const uint32_t hi = (uint32_t)(buf >> 32);
-                uint32_t ent = s_tab[hi >> (32 - P)];
-                if (!SINGLE && (ent >> 31)) {
-                    const uint32_t width = (ent >> 24) & 31u;
-                    const uint32_t idx2 = (hi << P) >> (32 - width);
-                    ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
-                }
-                const uint32_t t = (ent >> 8) & 31u;
-                slot[cnt] = (uint8_t)ent;
-                slot[cnt + 1] = (uint8_t)(ent >> 16);
-                cnt += 1u + ((ent >> 13) & 1u);
-                p += t;
-                buf <<= t;
-                q -= (int)t;
-                if (q < 32) {
-                    buf |= ((uint64_t)n2) << (32 - q);
-                    q += 32;
-                    n2 = n3;
-                    n3 = n4;
-                    n4 = 0;
-                }
-            }
-            // last few codewords of the segment, one at a time
-            while (p < limit) {
-                const uint32_t hi = (uint32_t)(buf >> 32);
-                uint32_t ent = s_tab[hi >> (32 - P)];
-                if (!SINGLE && (ent >> 31)) {
-                    const uint32_t width = (ent >> 24) & 31u;
-                    const uint32_t idx2 = (hi << P) >> (32 - width);
-                    ent = s_tab[l1_entries + (ent & 0xFFFFFFu) + idx2];
-                }
-                const uint32_t l = (ent >> 24) & 31u;
-                slot[cnt++] = (uint8_t)ent;
-                p += l;
-                buf <<= l;
-                q -= (int)l;
-                if (q < 32) {
-                    buf |= ((uint64_t)n2) << (32 - q);
-                    q += 32;
-                    n2 = n3;
-                    n3 = n4;
-                    n4 = 0;
-                }
-            }
-        }
-
-        // 2. scan + look-back
-        uint32_t tile_syms;
-        const uint32_t excl = block_excl_sum(cnt, s_wave, tile_syms);
-        // pull the slot into registers while wave 0 looks back
-        uint32_t R[NDW];
-        const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
-#pragma unroll
-        for (int j = 0; j < NDW; j++) R[j] = (4u * j < cnt) ? slot_w[j] : 0u;
-        if (threadIdx.x < 64) {
-            const uint64_t O = look_back(status, tile, tile_syms);
-            if (threadIdx.x == 0) s_base = O;
-        }
-        __syncthreads();
-        const uint64_t O = s_base;
-
-        // 3. compact into LDS, aligned like the output
-        const uint32_t a = (uint32_t)(O & 15u) + excl;
-#pragma unroll
-        for (int j = 0; j < NDW; j++) {
-            if (__ballot(4u * j < cnt) == 0) break;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint32_t i = 4u * j + b;
-                if (i < cnt) s_bytes[a + i] = (uint8_t)(R[j] >> (8 * b));
-            }
-        }
-        __syncthreads();
-
-        // 4. store owned bytes
-        const uint64_t lo = O;
-        uint64_t hi = O + tile_syms;
-        if (hi > n) hi = n;
-        if (hi > lo) {
-            const uint64_t base = O & ~15ull;
-            const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
-            for (uint32_t c = threadIdx.x; c < chunks; c += kDecThreads) {
-                const uint64_t g = base + 16ull * c;
-                if (g >= lo && g + 16 <= hi) {
-                    const u32x4 v = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(out + g));
-                } else {
-                    for (int i = 0; i < 16; i++)
-                        if (g + i >= lo && g + i < hi) out[g + i] = s_bytes[16 * c + i];
-                }
-            }
-        }
-        (void)lane;
-        __syncthreads();
+        if (tile >= buf.num_tiles) break;
+        dec_tile<THREADS, NDW, SINGLE>(buf, tile, t, s_tab, s_bytes, s_wave, &s_base, wpl);
     }
 }
 
@@ -589,14 +675,26 @@ static int device_cus() {
     return cached;
 }
 
+constexpr size_t kMaxLds = 160 * 1024;
+
+// persistent grid: as many workgroups as fit (or blocks_per_cu per CU when set), at most `work`
 template <typename K>
-static int resident_blocks(K kernel, int threads, size_t dyn_lds) {
+static unsigned grid_for(K kernel, int threads, size_t dyn_lds, uint32_t blocks_per_cu,
+                         uint64_t work) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, dyn_lds) !=
             hipSuccess || per_cu < 1)
         per_cu = 1;
+    if (blocks_per_cu > 0 && (int)blocks_per_cu < per_cu) per_cu = (int)blocks_per_cu;
     const int cus = device_cus();
-    return per_cu * (cus > 0 ? cus : 1);
+    const uint64_t g = (uint64_t)per_cu * (uint64_t)(cus > 0 ? cus : 1);
+    return (unsigned)(work < g ? work : g);
+}
+
+template <typename K>
+static void allow_dynamic_lds(K kernel) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds - 1024);
 }
 
 static hgap_status launch_zero(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb,
@@ -608,31 +706,114 @@ static hgap_status launch_zero(uint32_t *a, uint64_t na, uint32_t *b, uint64_t n
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
-template <int NDW, bool SINGLE>
-static hgap_status launch_dec(const uint32_t *payload, uint32_t W, const uint32_t *gap,
-                              uint32_t G, const uint32_t *table, const hgap_dectable_info *info,
-                              uint8_t *out, uint64_t n, uint32_t tiles, uint32_t *ws,
-                              hipStream_t s) {
-    const uint32_t table_words = info->l1_entries + info->l2_entries;
-    const uint32_t slot_offset = (table_words + 3u) & ~3u;
-    // staging region + 48 bytes: the last 16-byte copy-out chunk may read up to 29 bytes past
-    // 256 x NDW words (the bytes past the tile's output are never stored)
-    const size_t lds = 4ull * slot_offset + 4ull * (kDecThreads * NDW + 12);
-    if (lds > 160 * 1024 - 64) return HGAP_ERR_UNSUPPORTED;
-    auto kern = dec_kernel<NDW, SINGLE>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
-        attr_set = true;
+static uint32_t window_per_lane(const hgap_launch_config &c) {
+    uint32_t w = (c.lookback_window + 63) / 64;
+    return w < 1 ? 1 : (w > kMaxWindowPerLane ? kMaxWindowPerLane : w);
+}
+
+template <int THREADS, int K>
+static hgap_status launch_enc_t(const EncBuffer &b, const EncTable &t, uint32_t *counter,
+                                uint32_t max_len, const hgap_launch_config &c, hipStream_t s) {
+    constexpr uint32_t tile = THREADS * K;
+    const uint32_t out_words = enc_out_words(tile, max_len);
+    const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
+    if (lds + 256 * 4 + 64 > kMaxLds) return HGAP_ERR_UNSUPPORTED;
+    auto kern = enc_kernel<THREADS, K>;
+    static bool attr = false;
+    if (!attr) {
+        allow_dynamic_lds(kern);
+        attr = true;
     }
-    const int grid = resident_blocks(kern, kDecThreads, lds);
-    const unsigned blocks = (unsigned)(tiles < (uint32_t)grid ? tiles : (uint32_t)grid);
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kDecThreads), lds, s, payload, W, gap, G, table,
-                       table_words, slot_offset, info->prefix_bits, info->l1_entries, out, n,
-                       tiles, ws,
-                       reinterpret_cast<uint64_t *>(ws + 4));
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words,
+                       window_per_lane(c));
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+static hgap_status launch_enc(const EncBuffer &b, const EncTable &t, uint32_t *counter,
+                              uint32_t max_len, const hgap_launch_config &c, hipStream_t s) {
+#define HGAP_ENC_CASE(T, K)                                                      \
+    if (c.enc_threads == T && c.enc_bytes_per_thread == K)                     \
+        return launch_enc_t<T, K>(b, t, counter, max_len, c, s);
+    HGAP_ENC_CASE(256, 16) HGAP_ENC_CASE(256, 32) HGAP_ENC_CASE(256, 64)
+    HGAP_ENC_CASE(512, 16) HGAP_ENC_CASE(512, 32) HGAP_ENC_CASE(512, 64)
+    HGAP_ENC_CASE(1024, 16) HGAP_ENC_CASE(1024, 32) HGAP_ENC_CASE(1024, 64)
+#undef HGAP_ENC_CASE
+    return HGAP_ERR_UNSUPPORTED;
+}
+
+// slot words per lane: maxsym symbols plus one byte written past the last (second-symbol
+// store), rounded to an odd word count so that lanes' slots start in different LDS banks
+static uint32_t dec_slot_words(uint32_t max_symbols) {
+    if (max_symbols + 1 <= 4 * 11) return 11;
+    if (max_symbols + 1 <= 4 * 17) return 17;
+    return 33;
+}
+
+template <int THREADS, int NDW, bool SINGLE>
+static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
+                                const hgap_dectable_info &info, uint32_t *counter,
+                                const hgap_launch_config &c, hipStream_t s) {
+    const uint32_t table_words = info.l1_entries + info.l2_entries;
+    const uint32_t slot_offset = (table_words + 3u) & ~3u;
+    const size_t lds = 4ull * slot_offset + dec_stage_bytes(THREADS, NDW);
+    if (lds + 64 > kMaxLds) return HGAP_ERR_UNSUPPORTED;
+    auto kern = dec_kernel<THREADS, NDW, SINGLE>;
+    static bool attr = false;
+    if (!attr) {
+        allow_dynamic_lds(kern);
+        attr = true;
+    }
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
+    DecTable t{info.prefix_bits, info.l1_entries};
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
+                       slot_offset, counter, window_per_lane(c));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+template <int THREADS>
+static hgap_status launch_dec_threads(const DecBuffer &b, const uint32_t *table,
+                                      const hgap_dectable_info &info, uint32_t *counter,
+                                      const hgap_launch_config &c, hipStream_t s) {
+    const bool single = info.l2_entries == 0;
+    switch (dec_slot_words(info.max_symbols_per_segment)) {
+    case 11:
+        return single ? launch_dec_t<THREADS, 11, true>(b, table, info, counter, c, s)
+                      : launch_dec_t<THREADS, 11, false>(b, table, info, counter, c, s);
+    case 17:
+        return single ? launch_dec_t<THREADS, 17, true>(b, table, info, counter, c, s)
+                      : launch_dec_t<THREADS, 17, false>(b, table, info, counter, c, s);
+    default:
+        return single ? launch_dec_t<THREADS, 33, true>(b, table, info, counter, c, s)
+                      : launch_dec_t<THREADS, 33, false>(b, table, info, counter, c, s);
+    }
+}
+
+static hgap_status launch_dec(const DecBuffer &b, const uint32_t *table,
+                              const hgap_dectable_info &info, uint32_t *counter,
+                              const hgap_launch_config &c, hipStream_t s) {
+    switch (c.dec_threads) {
+    case 256: return launch_dec_threads<256>(b, table, info, counter, c, s);
+    case 512: return launch_dec_threads<512>(b, table, info, counter, c, s);
+    case 1024: return launch_dec_threads<1024>(b, table, info, counter, c, s);
+    default: return HGAP_ERR_UNSUPPORTED;
+    }
+}
+
+static bool config_valid(const hgap_launch_config &c) {
+    const bool et = c.enc_threads == 256 || c.enc_threads == 512 || c.enc_threads == 1024;
+    const bool ek = c.enc_bytes_per_thread == 16 || c.enc_bytes_per_thread == 32 ||
+                    c.enc_bytes_per_thread == 64;
+    const bool dt = c.dec_threads == 256 || c.dec_threads == 512 || c.dec_threads == 1024;
+    return et && ek && dt && c.lookback_window >= 1 && c.lookback_window <= 256 &&
+           c.prefix_bits <= HGAP_MAX_CODE_LEN;
+}
+
+static hgap_launch_config resolve(const hgap_launch_config *cfg) {
+    hgap_launch_config c;
+    hgap_default_config(&c);
+    if (cfg) c = *cfg;
+    return c;
 }
 
 }  // namespace hgap
@@ -643,6 +824,16 @@ extern "C" {
 
 int hgap_device_cu_count(void) { return device_cus(); }
 
+void hgap_default_config(hgap_launch_config *cfg) {
+    if (!cfg) return;
+    cfg->enc_threads = kDefaultEncThreads;
+    cfg->enc_bytes_per_thread = kDefaultEncBytesPerThread;
+    cfg->dec_threads = kDefaultDecThreads;
+    cfg->lookback_window = kDefaultLookbackWindow;
+    cfg->blocks_per_cu = 0;
+    cfg->prefix_bits = 0;
+}
+
 hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
                            hgap_stream_t stream) {
     if (!d_hist || (n > 0 && !d_in)) return HGAP_ERR_INVALID_ARG;
@@ -650,25 +841,26 @@ hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
     hipStream_t s = static_cast<hipStream_t>(stream);
     hgap_status st = launch_zero(d_hist, 256, nullptr, 0, s);
     if (st != HGAP_OK || n == 0) return st;
-    static int grid = 0;
-    if (grid == 0) grid = resident_blocks(hist_kernel, kHistThreads, 0);
     uint64_t want = ((n >> 4) + kHistThreads * 4 - 1) / (kHistThreads * 4);
     if (want < 1) want = 1;
-    const unsigned blocks = (unsigned)(want < (uint64_t)grid ? want : (uint64_t)grid);
+    const unsigned blocks = grid_for(hist_kernel, kHistThreads, 0, 0, want);
     hipLaunchKernelGGL(hist_kernel, dim3(blocks), dim3(kHistThreads), 0, s,
                        static_cast<const uint8_t *>(d_in), n, d_hist);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
 size_t hgap_encode_workspace_bytes(uint64_t n) {
-    const uint64_t tiles = (n + kEncTileBytes - 1) / kEncTileBytes;
+    const uint64_t tiles = (n + kMinEncTileBytes - 1) / kMinEncTileBytes;
     return 16 + 8 * tiles;
 }
 
-hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
-                        const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
-                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                            const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                            void *d_workspace, size_t workspace_bytes,
+                            const hgap_launch_config *cfg, hgap_stream_t stream) {
     if (!cb || !sizes) return HGAP_ERR_INVALID_ARG;
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
     if (n == 0) return (sizes->payload_words == 0 && sizes->gap_elements == 0) ? HGAP_OK
                                                                             : HGAP_ERR_SIZE;
     if (!d_in || !d_payload || !d_gap || !d_workspace) return HGAP_ERR_INVALID_ARG;
@@ -677,42 +869,57 @@ hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
         return HGAP_ERR_INVALID_ARG;
     if (n > 0xFFFFFFFFull) return HGAP_ERR_SIZE;
     if (workspace_bytes < hgap_encode_workspace_bytes(n)) return HGAP_ERR_WORKSPACE;
-    const uint64_t tiles64 = (n + kEncTileBytes - 1) / kEncTileBytes;
-    const uint32_t tiles = (uint32_t)tiles64;
-    const uint32_t gap_words = (uint32_t)(((uint64_t)sizes->gap_elements + 7) / 8);
     if ((uint64_t)sizes->payload_words * 32 < sizes->bits ||
         (sizes->bits > 0 && (uint64_t)sizes->payload_words * 32 >= sizes->bits + 32))
         return HGAP_ERR_SIZE;
     EncTable t;
+    uint32_t max_len = 1;
     for (int b = 0; b < 256; b++) {
         const uint32_t l = cb->length[b];
         if (l > HGAP_MAX_CODE_LEN || (l > 0 && (cb->code[b] >> l) != 0)) return HGAP_ERR_CODEBOOK;
         t.e[b] = (cb->code[b] << 5) | l;
+        if (l > max_len) max_len = l;
     }
+    const uint64_t tile = (uint64_t)c.enc_threads * c.enc_bytes_per_thread;
+    const uint64_t tiles = (n + tile - 1) / tile;
+    const uint32_t gap_words = (uint32_t)(((uint64_t)sizes->gap_elements + 7) / 8);
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t *ws = static_cast<uint32_t *>(d_workspace);
-    hgap_status st = launch_zero(d_gap, gap_words, ws, 4 + 2 * tiles64, s);
+    hgap_status st = launch_zero(d_gap, gap_words, ws, 4 + 2 * tiles, s);
     if (st != HGAP_OK) return st;
-    static int grid = 0;
-    if (grid == 0) grid = resident_blocks(enc_kernel, kEncThreads, 0);
-    const unsigned blocks = (unsigned)(tiles < (uint32_t)grid ? tiles : (uint32_t)grid);
-    hipLaunchKernelGGL(enc_kernel, dim3(blocks), dim3(kEncThreads), 0, s,
-                       static_cast<const uint8_t *>(d_in), n, tiles, t, d_payload,
-                       sizes->payload_words, d_gap, gap_words, ws,
-                       reinterpret_cast<uint64_t *>(ws + 4));
-    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+    EncBuffer b;
+    b.in = static_cast<const uint8_t *>(d_in);
+    b.n = n;
+    b.payload = d_payload;
+    b.payload_words = sizes->payload_words;
+    b.gap_words = gap_words;
+    b.gap = d_gap;
+    b.status = reinterpret_cast<uint64_t *>(ws + 4);
+    b.num_tiles = (uint32_t)tiles;
+    b.reserved = 0;
+    return launch_enc(b, t, ws, max_len, c, s);
+}
+
+hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                        const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+    return hgap_encode_cfg(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
+                           nullptr, stream);
 }
 
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements) {
-    const uint64_t tiles = ((uint64_t)gap_elements + kDecThreads - 1) / kDecThreads;
+    const uint64_t tiles = ((uint64_t)gap_elements + kMinDecThreads - 1) / kMinDecThreads;
     return 16 + 8 * tiles;
 }
 
-hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const uint32_t *d_gap,
-                        uint32_t gap_elements, const void *d_table,
-                        const hgap_dectable_info *info, void *d_out, uint64_t n,
-                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
+                            const uint32_t *d_gap, uint32_t gap_elements, const void *d_table,
+                            const hgap_dectable_info *info, void *d_out, uint64_t n,
+                            void *d_workspace, size_t workspace_bytes,
+                            const hgap_launch_config *cfg, hgap_stream_t stream) {
     if (!info) return HGAP_ERR_INVALID_ARG;
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
     if (n == 0) return HGAP_OK;
     if (!d_payload || !d_gap || !d_table || !d_out || !d_workspace) return HGAP_ERR_INVALID_ARG;
     if (!aligned16(d_payload) || !aligned16(d_gap) || !aligned16(d_table) || !aligned16(d_out) ||
@@ -723,33 +930,30 @@ hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const
         info->l1_entries != (1u << info->prefix_bits) || info->min_length < 1)
         return HGAP_ERR_CODEBOOK;
     if (workspace_bytes < hgap_decode_workspace_bytes(gap_elements)) return HGAP_ERR_WORKSPACE;
-    const uint32_t tiles = (uint32_t)(((uint64_t)gap_elements + kDecThreads - 1) / kDecThreads);
+    const uint64_t tiles = ((uint64_t)gap_elements + c.dec_threads - 1) / c.dec_threads;
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t *ws = static_cast<uint32_t *>(d_workspace);
     hgap_status st = launch_zero(ws, 4 + 2ull * tiles, nullptr, 0, s);
     if (st != HGAP_OK) return st;
-    const uint32_t *tab = static_cast<const uint32_t *>(d_table);
-    uint8_t *out = static_cast<uint8_t *>(d_out);
-    const bool single = info->l2_entries == 0;
-    const uint32_t maxsym = info->max_symbols_per_segment;
-    // slot widths (words) are odd so that lanes' slots start in different LDS banks
-    // a slot holds maxsym symbols plus one byte written past the last (second-symbol store)
-    if (maxsym + 1 <= 4 * 11) {
-        return single ? launch_dec<11, true>(d_payload, payload_words, d_gap, gap_elements, tab,
-                                             info, out, n, tiles, ws, s)
-                      : launch_dec<11, false>(d_payload, payload_words, d_gap, gap_elements,
-                                              tab, info, out, n, tiles, ws, s);
-    }
-    if (maxsym + 1 <= 4 * 17) {
-        return single ? launch_dec<17, true>(d_payload, payload_words, d_gap, gap_elements, tab,
-                                             info, out, n, tiles, ws, s)
-                      : launch_dec<17, false>(d_payload, payload_words, d_gap, gap_elements,
-                                              tab, info, out, n, tiles, ws, s);
-    }
-    return single ? launch_dec<33, true>(d_payload, payload_words, d_gap, gap_elements, tab, info,
-                                         out, n, tiles, ws, s)
-                  : launch_dec<33, false>(d_payload, payload_words, d_gap, gap_elements, tab,
-                                          info, out, n, tiles, ws, s);
+    DecBuffer b;
+    b.payload = d_payload;
+    b.payload_words = payload_words;
+    b.gap_elements = gap_elements;
+    b.gap = d_gap;
+    b.out = static_cast<uint8_t *>(d_out);
+    b.n = n;
+    b.status = reinterpret_cast<uint64_t *>(ws + 4);
+    b.num_tiles = (uint32_t)tiles;
+    b.reserved = 0;
+    return launch_dec(b, static_cast<const uint32_t *>(d_table), *info, ws, c, s);
+}
+
+hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const uint32_t *d_gap,
+                        uint32_t gap_elements, const void *d_table,
+                        const hgap_dectable_info *info, void *d_out, uint64_t n,
+                        void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
+    return hgap_decode_cfg(d_payload, payload_words, d_gap, gap_elements, d_table, info, d_out, n,
+                           d_workspace, workspace_bytes, nullptr, stream);
 }
 
 hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,

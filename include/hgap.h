@@ -75,7 +75,19 @@ typedef struct hgap_dectable_info {
     uint32_t symbol_count;
 } hgap_dectable_info;
 
+/* Launch shape of the device steps.  hgap_default_config() gives the tuned defaults; the
+ * autotuner (huffman_coding_gap_arrays_amd/tuner.py) sweeps these fields. */
+typedef struct hgap_launch_config {
+    uint32_t enc_threads;          /* encoder workgroup size: 256, 512 or 1024 */
+    uint32_t enc_bytes_per_thread; /* input bytes per encoder thread: 16, 32 or 64 */
+    uint32_t dec_threads;          /* decoder workgroup size = segments per tile: 256/512/1024 */
+    uint32_t lookback_window;      /* predecessor tiles read per look-back round: 1..256 */
+    uint32_t blocks_per_cu;        /* persistent workgroups per CU; 0 = as many as fit */
+    uint32_t prefix_bits;          /* decode-table first-level width; 0 = library default */
+} hgap_launch_config;
+
 const char *hgap_status_string(int status);
+void hgap_default_config(hgap_launch_config *cfg);
 /* Device (ordinal) properties used by the launchers; -1 when no usable device. */
 int hgap_device_cu_count(void);
 
@@ -118,7 +130,8 @@ hgap_status hgap_stream_parse_header(const uint8_t *src, size_t size, hgap_codeb
 hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
                            hgap_stream_t stream);
 
-/* Encoder workspace: tile counter + one 64-bit look-back status word per input tile. */
+/* Encoder workspace: tile counter + one 64-bit look-back status word per input tile (sized
+ * for the smallest tile any configuration uses). */
 size_t hgap_encode_workspace_bytes(uint64_t n);
 /* Writes payload[W] and gap[ceil(G/8)] for d_in[0..n) with codebook `cb`; sizes must come from
  * hgap_encode_sizes for the same input.  Replaces encode() (encoder/src/encoder.cu:382-457):
@@ -126,6 +139,13 @@ size_t hgap_encode_workspace_bytes(uint64_t n);
 hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
                         const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
+
+/* hgap_encode with an explicit launch shape (NULL = defaults); HGAP_ERR_UNSUPPORTED when the
+ * shape does not fit the device (e.g. its LDS image for this codebook's longest code). */
+hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                            const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                            void *d_workspace, size_t workspace_bytes,
+                            const hgap_launch_config *cfg, hgap_stream_t stream);
 
 /* Decoder workspace: tile counter + one 64-bit look-back status word per tile of segments. */
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements);
@@ -136,6 +156,13 @@ hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const
                         uint32_t gap_elements, const void *d_table,
                         const hgap_dectable_info *info, void *d_out, uint64_t n,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
+
+/* hgap_decode with an explicit launch shape (NULL = defaults). */
+hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
+                            const uint32_t *d_gap, uint32_t gap_elements, const void *d_table,
+                            const hgap_dectable_info *info, void *d_out, uint64_t n,
+                            void *d_workspace, size_t workspace_bytes,
+                            const hgap_launch_config *cfg, hgap_stream_t stream);
 
 /* Synthetic data for tests and benchmarks: byte i = perm[rank], rank = first r with
  * (splitmix64(seed + (start+i+1)*0x9E3779B97F4A7C15) >> 11) < cdf[r]  (cdf[255] = 2^53). */

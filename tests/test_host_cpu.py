@@ -224,8 +224,21 @@ def test_device_entry_points_validate_before_launch(capi):
     assert capi.hgap_decode(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 8, None) == 4
     assert capi.hgap_histogram(ctypes.c_void_p(3), 10, ctypes.c_void_p(16), None) == 1
     assert capi.hgap_encode_workspace_bytes(0) == 16
-    assert capi.hgap_encode_workspace_bytes(8193) == 16 + 2 * 8
-    assert capi.hgap_decode_workspace_bytes(257) == 16 + 2 * 8
+    assert capi.hgap_encode_workspace_bytes(8193) == 16 + 3 * 8  # smallest tile: 4 KiB
+    assert capi.hgap_decode_workspace_bytes(257) == 16 + 2 * 8  # smallest tile: 256 segments
+    # launch shapes outside the supported set are refused before anything is launched
+    from huffman_coding_gap_arrays_amd._capi import LaunchConfig, default_config
+
+    good = default_config()
+    assert good.enc_threads in (256, 512, 1024) and good.lookback_window <= 256
+    for field, value in (("enc_threads", 300), ("enc_bytes_per_thread", 8), ("dec_threads", 128),
+                         ("lookback_window", 0), ("lookback_window", 512), ("prefix_bits", 17)):
+        bad = LaunchConfig.from_dict({**good.as_dict(), field: value})
+        assert capi.hgap_encode_cfg(ctypes.c_void_p(16), 100, ctypes.byref(cb), ctypes.byref(sz),
+                                    ctypes.c_void_p(32), ctypes.c_void_p(48),
+                                    ctypes.c_void_p(64), 1 << 20, ctypes.byref(bad), None) == 1
+        assert capi.hgap_decode_cfg(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 4096,
+                                    ctypes.byref(bad), None) == 1
 
 
 # ---- reference-named entry points (include/hgap_compat.h) ---------------------------------
