@@ -683,23 +683,39 @@ static unsigned grid_for(K kernel, int threads, size_t dyn_lds, uint32_t blocks_
                          uint64_t work) {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, dyn_lds) !=
-            hipSuccess || per_cu < 1)
+            hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
         per_cu = 1;
+    }
     if (blocks_per_cu > 0 && (int)blocks_per_cu < per_cu) per_cu = (int)blocks_per_cu;
     const int cus = device_cus();
     const uint64_t g = (uint64_t)per_cu * (uint64_t)(cus > 0 ? cus : 1);
     return (unsigned)(work < g ? work : g);
 }
 
+// Raises the kernel's dynamic-LDS limit to what this launch needs.  Returns false (with HIP's
+// error state cleared) when static + dynamic LDS exceed the CU's 160 KiB.
 template <typename K>
-static void allow_dynamic_lds(K kernel) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds - 1024);
+static bool allow_dynamic_lds(K kernel, size_t dyn_bytes) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kernel)) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (attr.sharedSizeBytes + dyn_bytes > kMaxLds) return false;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(kMaxLds - attr.sharedSizeBytes)) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
 }
 
 static hgap_status launch_zero(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb,
                                hipStream_t s) {
     if (na + nb == 0) return HGAP_OK;
+    (void)hipGetLastError();  // report only this launch's error below
     uint64_t blocks = (na + nb + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, na, b, nb);
@@ -717,13 +733,8 @@ static hgap_status launch_enc_t(const EncBuffer &b, const EncTable &t, uint32_t 
     constexpr uint32_t tile = THREADS * K;
     const uint32_t out_words = enc_out_words(tile, max_len);
     const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
-    if (lds + 256 * 4 + 64 > kMaxLds) return HGAP_ERR_UNSUPPORTED;
     auto kern = enc_kernel<THREADS, K>;
-    static bool attr = false;
-    if (!attr) {
-        allow_dynamic_lds(kern);
-        attr = true;
-    }
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words,
                        window_per_lane(c));
@@ -757,13 +768,8 @@ static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
     const uint32_t table_words = info.l1_entries + info.l2_entries;
     const uint32_t slot_offset = (table_words + 3u) & ~3u;
     const size_t lds = 4ull * slot_offset + dec_stage_bytes(THREADS, NDW);
-    if (lds + 64 > kMaxLds) return HGAP_ERR_UNSUPPORTED;
     auto kern = dec_kernel<THREADS, NDW, SINGLE>;
-    static bool attr = false;
-    if (!attr) {
-        allow_dynamic_lds(kern);
-        attr = true;
-    }
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
     DecTable t{info.prefix_bits, info.l1_entries};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
