@@ -189,9 +189,14 @@ def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, pa
         gap = _words(sizes.gap_words, dev)
     wsb = _lib().hgap_encode_workspace_bytes(n)
     ws = (workspace or Workspace(dev)).get(wsb)
-    check(_lib().hgap_encode_cfg(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
-                                 _ptr(payload), _ptr(gap), _ptr(ws), ws.numel(), _cfg_ref(config),
-                                 _stream(stream)), "hgap_encode")
+    if config is None:  # default shape, falling back to smaller tiles when it does not fit
+        check(_lib().hgap_encode(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
+                                 _ptr(payload), _ptr(gap), _ptr(ws), ws.numel(), _stream(stream)),
+              "hgap_encode")
+    else:
+        check(_lib().hgap_encode_cfg(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
+                                     _ptr(payload), _ptr(gap), _ptr(ws), ws.numel(),
+                                     _cfg_ref(config), _stream(stream)), "hgap_encode")
     return Encoded(n=n, codebook=cb, sizes=sizes, payload=payload, gap=gap)
 
 
@@ -221,10 +226,15 @@ def decode(payload, gap, payload_words, gap_elements, table, n, out=None, stream
         out = torch.empty(max(int(n), 16), dtype=torch.uint8, device=dev)
     wsb = _lib().hgap_decode_workspace_bytes(gap_elements)
     ws = (workspace or Workspace(dev)).get(wsb)
-    check(_lib().hgap_decode_cfg(_ptr(payload), payload_words, _ptr(gap), gap_elements,
+    if config is None:  # default shape, falling back to smaller tiles when it does not fit
+        check(_lib().hgap_decode(_ptr(payload), payload_words, _ptr(gap), gap_elements,
                                  _ptr(table.table), ctypes.byref(table.info), _ptr(out), n,
-                                 _ptr(ws), ws.numel(), _cfg_ref(config), _stream(stream)),
-          "hgap_decode")
+                                 _ptr(ws), ws.numel(), _stream(stream)), "hgap_decode")
+    else:
+        check(_lib().hgap_decode_cfg(_ptr(payload), payload_words, _ptr(gap), gap_elements,
+                                     _ptr(table.table), ctypes.byref(table.info), _ptr(out), n,
+                                     _ptr(ws), ws.numel(), _cfg_ref(config), _stream(stream)),
+              "hgap_decode")
     return out[:n]
 
 

@@ -906,11 +906,20 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
     return launch_enc(b, t, ws, max_len, c, s);
 }
 
+// Default shape first; when its LDS image does not fit (long codes), smaller tiles.
 hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
                         const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
-    return hgap_encode_cfg(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
-                           nullptr, stream);
+    hgap_launch_config c;
+    hgap_default_config(&c);
+    hgap_status st = HGAP_ERR_UNSUPPORTED;
+    for (uint32_t threads = c.enc_threads; threads >= 256; threads /= 2) {
+        c.enc_threads = threads;
+        st = hgap_encode_cfg(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
+                             &c, stream);
+        if (st != HGAP_ERR_UNSUPPORTED) break;
+    }
+    return st;
 }
 
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements) {
@@ -954,12 +963,21 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     return launch_dec(b, static_cast<const uint32_t *>(d_table), *info, ws, c, s);
 }
 
+// Default shape first; when table + staging do not fit in LDS, fewer segments per tile.
 hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const uint32_t *d_gap,
                         uint32_t gap_elements, const void *d_table,
                         const hgap_dectable_info *info, void *d_out, uint64_t n,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
-    return hgap_decode_cfg(d_payload, payload_words, d_gap, gap_elements, d_table, info, d_out, n,
-                           d_workspace, workspace_bytes, nullptr, stream);
+    hgap_launch_config c;
+    hgap_default_config(&c);
+    hgap_status st = HGAP_ERR_UNSUPPORTED;
+    for (uint32_t threads = c.dec_threads; threads >= 256; threads /= 2) {
+        c.dec_threads = threads;
+        st = hgap_decode_cfg(d_payload, payload_words, d_gap, gap_elements, d_table, info, d_out,
+                             n, d_workspace, workspace_bytes, &c, stream);
+        if (st != HGAP_ERR_UNSUPPORTED) break;
+    }
+    return st;
 }
 
 hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,

@@ -132,6 +132,7 @@ def test_decode_prefix_splits(codec, orc, prefix):
     assert (info.l2_entries == 0) == (prefix >= info.max_length)
     ndw = 11 if info.max_symbols_per_segment + 1 <= 44 else (
         17 if info.max_symbols_per_segment + 1 <= 68 else 33)
+    # the default entry point falls back to 256-segment tiles before refusing
     lds = 4 * ((info.l1_entries + info.l2_entries + 3) // 4 * 4) + 4 * (256 * ndw + 12)
     if lds > 160 * 1024 - 64:
         with pytest.raises(codec.HgapError) as err:
