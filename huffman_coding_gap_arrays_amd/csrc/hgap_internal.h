@@ -14,10 +14,11 @@ constexpr uint32_t kDefaultPrefixBits = 11;
 // bytes per thread; decoder tile = threads segments of 128 bits.  Bigger tiles mean fewer
 // look-back participants; the look-back window is the number of predecessor status words read
 // per round.
-constexpr uint32_t kDefaultEncThreads = 1024;
-constexpr uint32_t kDefaultEncBytesPerThread = 32;
-constexpr uint32_t kDefaultDecThreads = 1024;
-constexpr uint32_t kDefaultLookbackWindow = 256;
+// (tuned on the 1 GiB Zipf(1.1) buffer with huffman_coding_gap_arrays_amd/tuner.py)
+constexpr uint32_t kDefaultEncThreads = 512;
+constexpr uint32_t kDefaultEncBytesPerThread = 64;
+constexpr uint32_t kDefaultDecThreads = 512;
+constexpr uint32_t kDefaultLookbackWindow = 64;
 // smallest tiles any configuration uses (they size the look-back workspaces)
 constexpr uint32_t kMinEncTileBytes = 256 * 16;
 constexpr uint32_t kMinDecThreads = 256;

@@ -222,12 +222,12 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
 //      and sums the lengths;
 //   2. workgroup scan of the bit counts, then decoupled look-back for the tile's global bit
 //      offset E;
-//   3. each thread packs its codes (two at a time) MSB-first into 32-bit words of an LDS tile
-//      buffer whose word 0 is global word (E/32) rounded down to a multiple of 4; a thread's
-//      first and last words may be shared with its neighbours (LDS atomic OR), all others are
-//      plain LDS stores.  When a completed word closes a 128-bit segment and a code spills past
-//      it, the spill (< 16) is that segment's gap nibble (encoder.cu:307-312), OR-ed into an
-//      LDS copy of the gap words;
+//   3. each thread packs its codes (two at a time) MSB-first into 32-bit words of a zeroed LDS
+//      tile image whose word 0 is global word (E/32) rounded down to a multiple of 4, with LDS
+//      atomic OR (a thread's first and last words may be shared with its neighbours).  When a
+//      completed word closes a 128-bit segment and a code spills past it, the spill (< 16) is
+//      that segment's gap nibble (encoder.cu:307-312), OR-ed into an LDS copy of the gap
+//      words;
 //   4. the tile owns global words [E/32, (E+T)/32) (the last tile also the final partial word);
 //      its first word also holds the last E%32 bits of the previous tile, which are rebuilt
 //      here from the previous tile's last input bytes instead of waiting for that tile;
@@ -317,7 +317,6 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         uint32_t w = (uint32_t)((B >> 5) - base_w);
         int nacc = (int)(B & 31);
         uint64_t acc = 0;
-        bool first = true;
 #pragma unroll
         for (int j = 0; j < K; j += 2) {
             const uint32_t l1 = e[j] & 31u, l2 = e[j + 1] & 31u;
@@ -326,12 +325,9 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
             nacc += (int)(l1 + l2);
             if (nacc >= 32) {
                 nacc -= 32;
-                const uint32_t word = (uint32_t)(acc >> nacc);
-                if (first)
-                    atomicOr(&s_out[w], word);
-                else
-                    s_out[w] = word;
-                first = false;
+                // the zeroed image takes an OR for every word: only a thread's first word is
+                // shared with its predecessor, and OR needs no branch on which one it is
+                atomicOr(&s_out[w], (uint32_t)(acc >> nacc));
                 if ((w & 3u) == 3u && nacc != 0) {
                     // nacc bits lie past the segment boundary: all of them belong to the second
                     // code if it straddles, else the first code straddles by nacc - l2 (0 when
@@ -465,11 +461,35 @@ __device__ __forceinline__ uint32_t dec_lookup(const uint32_t *s_tab, uint32_t h
     return ent;
 }
 
+// One decode step: look up the next first-level index; returns the table entry.
+#define HGAP_DEC_STEP(ENT)                                                                   \
+    const uint32_t ENT = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
+
 template <int THREADS, int NDW, bool SINGLE>
 __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, DecTable t,
-                                         const uint32_t *s_tab, uint8_t *s_bytes,
-                                         uint32_t *s_wave, uint64_t *s_base, uint32_t wpl) {
+                                         const uint32_t *s_tab, uint32_t *s_pay,
+                                         uint8_t *s_bytes, uint32_t *s_wave, uint64_t *s_base,
+                                         uint32_t wpl) {
     const uint32_t P = t.prefix_bits;
+    // 0. stage the tile's payload words (THREADS segments x 4 words + the word after them)
+    const uint64_t w_base = 4ull * tile * THREADS;
+    {
+        const uint64_t wi = w_base + 4ull * threadIdx.x;
+        u32x4 q = {0u, 0u, 0u, 0u};
+        if (wi + 4 <= buf.payload_words) {
+            q = *reinterpret_cast<const u32x4 *>(buf.payload + wi);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) q[i] = wi + i < buf.payload_words ? buf.payload[wi + i] : 0u;
+        }
+        *reinterpret_cast<u32x4 *>(s_pay + 4 * threadIdx.x) = q;
+        if (threadIdx.x < 8) {
+            const uint64_t x = w_base + 4ull * THREADS + threadIdx.x;
+            s_pay[4 * THREADS + threadIdx.x] = x < buf.payload_words ? buf.payload[x] : 0u;
+        }
+    }
+    __syncthreads();
+
     // 1. decode this lane's segment into its slot
     const uint32_t seg = tile * THREADS + threadIdx.x;
     uint32_t cnt = 0;
@@ -477,60 +497,55 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     if (seg < buf.gap_elements) {
         uint32_t start = 0;
         if (seg > 0) start = (buf.gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
-        const uint64_t w0i = 4ull * seg;
-        uint32_t w[5];
-        if (w0i + 5 <= buf.payload_words) {
-            const u32x4 q = *reinterpret_cast<const u32x4 *>(buf.payload + w0i);
-            w[0] = q[0];
-            w[1] = q[1];
-            w[2] = q[2];
-            w[3] = q[3];
-            w[4] = buf.payload[w0i + 4];
-        } else {
-#pragma unroll
-            for (int i = 0; i < 5; i++)
-                w[i] = (w0i + i < buf.payload_words) ? buf.payload[w0i + i] : 0u;
-        }
         const uint64_t avail = 32ull * buf.payload_words - 128ull * seg;
         const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
-        uint64_t bb = ((((uint64_t)w[0]) << 32) | w[1]) << start;
+        // bit buffer: the next q bits of the segment, MSB-aligned; `nx` = the following word
+        uint32_t k = 4 * threadIdx.x;
+        uint64_t bb = ((((uint64_t)s_pay[k]) << 32) | s_pay[k + 1]) << start;
         int q = 64 - (int)start;
-        uint32_t n2 = w[2], n3 = w[3], n4 = w[4];
+        uint32_t nx = s_pay[k + 2];
+        k += 3;
         uint32_t p = start;
-        // entries may hold two codewords; both start before `limit` while p + P <= limit
+#define HGAP_DEC_REFILL()                                                                     \
+    if (q < 32) {                                                                             \
+        bb |= ((uint64_t)nx) << (32 - q);                                                     \
+        q += 32;                                                                              \
+        nx = s_pay[k++];                                                                      \
+    }
+#define HGAP_DEC_PAIR_STEP()                                                                  \
+    {                                                                                         \
+        HGAP_DEC_STEP(ent)                                                                    \
+        const uint32_t adv = (ent >> 8) & 31u;                                                \
+        slot[cnt] = (uint8_t)ent;                                                             \
+        slot[cnt + 1] = (uint8_t)(ent >> 16);                                                 \
+        cnt += 1u + ((ent >> 13) & 1u);                                                       \
+        p += adv;                                                                             \
+        bb <<= adv;                                                                           \
+        q -= (int)adv;                                                                        \
+    }
+        // entries may hold two codewords; both start before `limit` while p + P <= limit.
+        // A step consumes at most 16 bits, so one refill every two steps keeps >= 16 bits.
+        while (p + 2 * P <= limit) {
+            HGAP_DEC_PAIR_STEP()
+            HGAP_DEC_PAIR_STEP()
+            HGAP_DEC_REFILL()
+        }
         while (p + P <= limit) {
-            const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
-            const uint32_t adv = (ent >> 8) & 31u;
-            slot[cnt] = (uint8_t)ent;
-            slot[cnt + 1] = (uint8_t)(ent >> 16);
-            cnt += 1u + ((ent >> 13) & 1u);
-            p += adv;
-            bb <<= adv;
-            q -= (int)adv;
-            if (q < 32) {
-                bb |= ((uint64_t)n2) << (32 - q);
-                q += 32;
-                n2 = n3;
-                n3 = n4;
-                n4 = 0;
-            }
+            HGAP_DEC_PAIR_STEP()
+            HGAP_DEC_REFILL()
         }
         // last few codewords of the segment, one at a time
         while (p < limit) {
-            const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
+            HGAP_DEC_STEP(ent)
             const uint32_t adv = (ent >> 24) & 31u;
             slot[cnt++] = (uint8_t)ent;
             p += adv;
             bb <<= adv;
             q -= (int)adv;
-            if (q < 32) {
-                bb |= ((uint64_t)n2) << (32 - q);
-                q += 32;
-                n2 = n3;
-                n3 = n4;
-                n4 = 0;
-            }
+            HGAP_DEC_REFILL()
         }
+#undef HGAP_DEC_PAIR_STEP
+#undef HGAP_DEC_REFILL
     }
 
     // 2. scan + look-back
@@ -540,7 +555,12 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     uint32_t R[NDW];
     const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
 #pragma unroll
-    for (int j = 0; j < NDW; j++) R[j] = (4u * j < cnt) ? slot_w[j] : 0u;
+    for (int j = 0; j < NDW; j++) R[j] = 0u;
+#pragma unroll
+    for (int j = 0; j < NDW; j++) {
+        if (__ballot(4u * j < cnt) == 0) break;
+        if (4u * j < cnt) R[j] = slot_w[j];
+    }
     if (threadIdx.x < 64) {
         const uint64_t O = look_back(buf.status, tile, tile_syms, wpl);
         if (threadIdx.x == 0) *s_base = O;
@@ -548,15 +568,30 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     __syncthreads();
     const uint64_t O = *s_base;
 
-    // 3. compact into LDS, aligned like the output
+    // 3. compact into LDS, aligned like the output: whole destination words with one store,
+    // the (at most two) words shared with the neighbouring lanes byte by byte
     const uint32_t a = (uint32_t)(O & 15u) + excl;
+    const uint32_t end = a + cnt;
+    const uint32_t m = a & 3u;
+    const uint32_t d0 = a >> 2;
+    uint32_t *s_words = reinterpret_cast<uint32_t *>(s_bytes);
 #pragma unroll
-    for (int j = 0; j < NDW; j++) {
-        if (__ballot(4u * j < cnt) == 0) break;
+    for (int j = 0; j < NDW + 1; j++) {
+        const uint32_t d = d0 + j;
+        if (__ballot(cnt > 0 && 4 * d < end) == 0) break;
+        const uint32_t cur = j < NDW ? R[j] : 0u;
+        const uint32_t prev = j > 0 ? R[j - 1] : 0u;
+        const uint32_t v = m ? __builtin_amdgcn_alignbyte(cur, prev, 4u - m) : cur;
+        const uint32_t lo = 4 * d > a ? 4 * d : a;
+        const uint32_t hi = 4 * d + 4 < end ? 4 * d + 4 : end;
+        if (lo < hi) {
+            if (lo == 4 * d && hi == 4 * d + 4) {
+                s_words[d] = v;
+            } else {
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const uint32_t i = 4u * j + b;
-            if (i < cnt) s_bytes[a + i] = (uint8_t)(R[j] >> (8 * b));
+                for (uint32_t b = 0; b < 4; b++)
+                    if (4 * d + b >= lo && 4 * d + b < hi) s_bytes[4 * d + b] = (uint8_t)(v >> (8 * b));
+            }
         }
     }
     __syncthreads();
@@ -581,12 +616,19 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     }
     __syncthreads();
 }
+#undef HGAP_DEC_STEP
 
-// LDS bytes of the decode tile staging region: THREADS slots of NDW words, plus 48 bytes
-// because the last 16-byte copy-out chunk may read up to 29 bytes past THREADS x NDW words
-// (bytes past the tile's output are never stored)
+// LDS words of the tile's staged payload: 4 per segment, plus the words a lane's bit buffer may
+// prefetch past its last segment (16-byte multiple)
+__host__ __device__ constexpr uint32_t dec_payload_words(uint32_t threads) {
+    return 4u * threads + 8u;
+}
+
+// LDS bytes of the decode tile staging region: the staged payload, then THREADS slots of NDW
+// words plus 48 bytes (the last 16-byte copy-out chunk may read up to 29 bytes past THREADS x
+// NDW words; bytes past the tile's output are never stored)
 __host__ __device__ constexpr uint32_t dec_stage_bytes(uint32_t threads, uint32_t ndw) {
-    return 4u * (threads * ndw + 12u);
+    return 4u * (dec_payload_words(threads) + threads * ndw + 12u);
 }
 
 template <int THREADS, int NDW, bool SINGLE>
@@ -600,14 +642,15 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_tile;
     uint32_t *s_tab = s_dyn;
-    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_dyn + slot_offset);  // 16-byte aligned
+    uint32_t *s_pay = s_dyn + slot_offset;  // 16-byte aligned; dec_payload_words(THREADS)
+    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_pay + dec_payload_words(THREADS));
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
     while (true) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= buf.num_tiles) break;
-        dec_tile<THREADS, NDW, SINGLE>(buf, tile, t, s_tab, s_bytes, s_wave, &s_base, wpl);
+        dec_tile<THREADS, NDW, SINGLE>(buf, tile, t, s_tab, s_pay, s_bytes, s_wave, &s_base, wpl);
     }
 }
 
