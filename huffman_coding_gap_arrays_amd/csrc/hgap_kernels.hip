@@ -447,6 +447,7 @@ struct DecBuffer {
 struct DecTable {
     uint32_t prefix_bits;
     uint32_t l1_entries;
+    uint32_t max_step;  // most bits one lookup can consume: max(prefix_bits, longest code)
 };
 
 template <bool SINGLE>
@@ -524,8 +525,10 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
         q -= (int)adv;                                                                        \
     }
         // entries may hold two codewords; both start before `limit` while p + P <= limit.
-        // A step consumes at most 16 bits, so one refill every two steps keeps >= 16 bits.
-        while (p + 2 * P <= limit) {
+        // Two steps per refill: the second step starts at most max_step bits later, so both
+        // of its codewords start before `limit` while p + max_step + P <= limit; a step
+        // consumes at most 16 bits, so one refill every two steps keeps >= 16 bits buffered.
+        while (p + t.max_step + P <= limit) {
             HGAP_DEC_PAIR_STEP()
             HGAP_DEC_PAIR_STEP()
             HGAP_DEC_REFILL()
@@ -814,7 +817,8 @@ static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
     auto kern = dec_kernel<THREADS, NDW, SINGLE>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
-    DecTable t{info.prefix_bits, info.l1_entries};
+    DecTable t{info.prefix_bits, info.l1_entries,
+               info.max_length > info.prefix_bits ? info.max_length : info.prefix_bits};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
                        slot_offset, counter, window_per_lane(c));
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
