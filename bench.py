@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
     ap.add_argument("--cpu-repeats", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096,
+                    help="buffers of the batched workload (0 = skip it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -165,6 +167,15 @@ def main():
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=load_traffic(args.config, kern),
                     alg_bytes_per_launch=stages[dom]["alg_bytes"])
+    batched = None
+    if args.batch_count > 0:
+        log("batched workload ...")
+        del out
+        from huffman_coding_gap_arrays_amd import batch
+
+        batched = batch.bench(count=args.batch_count)
+        batched["workload"] = (f"{args.batch_count} buffers, sizes log-uniform 4 KiB..64 MiB, "
+                               "Zipf s ~ U(0.6, 2.0), own permutation and codebook each")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -192,6 +203,7 @@ def main():
         "stages": per_stage,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "round_trip_exact": ok,
     }
     print(json.dumps(line), flush=True)

@@ -98,6 +98,35 @@ def default_config():
     return c
 
 
+class HistJob(ctypes.Structure):
+    """hgap_hist_job (one buffer of a batched histogram)."""
+
+    _fields_ = [("in_", ctypes.c_uint64), ("n", ctypes.c_uint64), ("hist", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint32 * 2), ("base", ctypes.c_uint64)]
+
+
+class EncJob(ctypes.Structure):
+    """hgap_enc_job (one buffer of a batched encode)."""
+
+    _fields_ = [("in_", ctypes.c_uint64), ("n", ctypes.c_uint64), ("payload", ctypes.c_uint64),
+                ("gap", ctypes.c_uint64), ("payload_words", ctypes.c_uint32),
+                ("gap_words", ctypes.c_uint32), ("table", ctypes.c_uint32),
+                ("max_len", ctypes.c_uint32), ("base", ctypes.c_uint32),
+                ("num_tiles", ctypes.c_uint32)]
+
+
+class DecJob(ctypes.Structure):
+    """hgap_dec_job (one buffer of a batched decode)."""
+
+    _fields_ = [("payload", ctypes.c_uint64), ("gap", ctypes.c_uint64), ("out", ctypes.c_uint64),
+                ("n", ctypes.c_uint64), ("table", ctypes.c_uint64),
+                ("payload_words", ctypes.c_uint32), ("gap_elements", ctypes.c_uint32),
+                ("prefix_bits", ctypes.c_uint32), ("l1_entries", ctypes.c_uint32),
+                ("table_words", ctypes.c_uint32), ("max_symbols", ctypes.c_uint32),
+                ("max_length", ctypes.c_uint32), ("base", ctypes.c_uint32),
+                ("num_tiles", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 class Symbol(ctypes.Structure):
     """struct Symbol (encoder/include/constants.hpp:34-38)."""
 
@@ -177,6 +206,15 @@ PROTOTYPES = {
         _INT,
         [_P, _U32, _P, _U32, _P, ctypes.POINTER(DectableInfo), _P, _U64, _P, _SZ, _P],
     ),
+    "hgap_batch_plan_histogram": (_U64, [_P, _U32]),
+    "hgap_batch_histogram": (_INT, [_P, _U32, _U64, _P]),
+    "hgap_codebook_build_batch": (_INT, [_P, _U32, _P, _U32]),
+    "hgap_encode_table": (_INT, [ctypes.POINTER(Codebook), _P]),
+    "hgap_batch_plan_encode": (_U32, [_P, _U32, ctypes.c_void_p]),
+    "hgap_batch_plan_decode": (_U32, [_P, _U32, ctypes.c_void_p]),
+    "hgap_batch_workspace_bytes": (_SZ, [_U32]),
+    "hgap_batch_encode": (_INT, [_P, _U32, _U32, _U32, _P, _P, _SZ, ctypes.c_void_p, _P]),
+    "hgap_batch_decode": (_INT, [_P, _U32, _U32, _U32, _U32, _P, _SZ, ctypes.c_void_p, _P]),
     "hgap_synth_fill": (_INT, [_P, _U64, _U64, _U64, _P, _P, _P]),
     # reference-named entry points (include/hgap_compat.h)
     "histgram": (None, [_P, _P, _U32]),

@@ -3,7 +3,9 @@
 //
 // Reference citations are file:line relative to the reference repository root.
 #include <cstdint>
+#include <atomic>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "hgap.h"
@@ -192,6 +194,30 @@ hgap_status hgap_codebook_build(const uint32_t hist[256], hgap_codebook *cb) {
         cb->order_length[t] = len[k - 1 - t];
     }
     return assign_codes(cb) ? HGAP_OK : HGAP_ERR_CODEBOOK;
+}
+
+// Codebooks for many histograms on a pool of host threads; each is independent and built
+// exactly like hgap_codebook_build.
+hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
+                                      uint32_t threads) {
+    if (count == 0) return HGAP_OK;
+    if (!hists || !cbs) return HGAP_ERR_INVALID_ARG;
+    uint32_t hw = std::thread::hardware_concurrency();
+    if (threads == 0) threads = hw ? hw : 1;
+    if (threads > count) threads = count;
+    std::atomic<uint32_t> next{0};
+    std::atomic<int> err{HGAP_OK};
+    auto work = [&]() {
+        for (uint32_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
+            const hgap_status st = hgap_codebook_build(hists + 256ull * i, &cbs[i]);
+            if (st != HGAP_OK) err.store(st);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; t++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    return (hgap_status)err.load();
 }
 
 hgap_status hgap_codebook_from_order(const uint8_t *order_symbol, const uint8_t *order_length,

@@ -658,6 +658,216 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
 }
 
 // ---------------------------------------------------------------------------------------------
+// Batched steps: many independent buffers, each with its own codebook or decode table, in one
+// persistent launch per step.  Job descriptors live in device memory (so a captured graph
+// replays the same launches).  Encode/decode jobs own look-back status words
+// [tile_base, tile_base + num_tiles); tiles are claimed in job order from one counter, so every
+// predecessor tile of a job is resident or finished.
+
+// job index owning global unit `g` (units = tiles or bytes), jobs sorted by base
+template <typename J>
+__device__ __forceinline__ uint32_t find_job(const J *jobs, uint32_t count, uint64_t g) {
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if ((uint64_t)jobs[mid].base <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct HistJobDev {
+    const uint8_t *in;
+    uint64_t n;
+    uint32_t *hist;
+    uint32_t reserved[2];
+    uint64_t base;  // first byte of this job in the batch's concatenation
+};
+
+// zero every job's 256 bins (one workgroup per job, strided)
+__global__ void hist_batch_zero_kernel(const HistJobDev *jobs, uint32_t count) {
+    for (uint32_t j = blockIdx.x; j < count; j += gridDim.x) jobs[j].hist[threadIdx.x] = 0u;
+}
+
+// Each workgroup counts one contiguous slice of the concatenated jobs; at every job boundary
+// inside its slice it flushes its 64 LDS bin copies into that job's histogram.
+__global__ __launch_bounds__(kHistThreads) void hist_batch_kernel(const HistJobDev *jobs,
+                                                                  uint32_t count,
+                                                                  uint64_t total) {
+    __shared__ uint32_t s_bins[kHistCopies * 256];
+    const uint32_t copy = lane_id();
+    uint32_t *mine = s_bins + copy * 256;
+    for (int i = threadIdx.x; i < kHistCopies * 256; i += kHistThreads) s_bins[i] = 0u;
+    const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
+    const uint64_t a = (uint64_t)blockIdx.x * per;
+    const uint64_t b = a + per < total ? a + per : total;
+    __syncthreads();
+    if (a >= b) return;
+    uint32_t j = find_job(jobs, count, a);
+    for (uint64_t pos = a; pos < b;) {
+        const HistJobDev job = jobs[j];
+        const uint64_t lo = pos - job.base;
+        const uint64_t hi = (b < job.base + job.n ? b : job.base + job.n) - job.base;
+        // [lo, hi) of this job: bytes up to the first 16-byte boundary, vectors, tail bytes
+        const uint64_t vlo = (lo + 15) & ~15ull, vhi = hi & ~15ull;
+        if (vlo >= vhi) {
+            for (uint64_t i = lo + threadIdx.x; i < hi; i += kHistThreads)
+                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+        } else {
+            for (uint64_t i = lo + threadIdx.x; i < vlo; i += kHistThreads)
+                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+            for (uint64_t i = vhi + threadIdx.x; i < hi; i += kHistThreads)
+                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+            const u32x4 *in4 = reinterpret_cast<const u32x4 *>(job.in);
+            for (uint64_t v = (vlo >> 4) + threadIdx.x; v < (vhi >> 4); v += kHistThreads) {
+                const u32x4 q = __builtin_nontemporal_load(&in4[v]);
+#pragma unroll
+                for (int k = 0; k < 16; k++)
+                    atomicAdd(&mine[((q[k >> 2] >> (8 * (k & 3))) & 0xFFu) ^ copy], 1u);
+            }
+        }
+        __syncthreads();
+        uint32_t sum = 0;
+        for (int c = 0; c < kHistCopies; c++) {
+            sum += s_bins[c * 256 + (threadIdx.x ^ c)];
+            s_bins[c * 256 + (threadIdx.x ^ c)] = 0u;
+        }
+        if (sum) atomicAdd(&job.hist[threadIdx.x], sum);
+        __syncthreads();
+        pos = job.base + hi;
+        j++;
+    }
+}
+
+struct EncJobDev {
+    const uint8_t *in;
+    uint64_t n;
+    uint32_t *payload;
+    uint32_t *gap;
+    uint32_t payload_words;
+    uint32_t gap_words;
+    uint32_t table;    // index of this job's 256-word code table
+    uint32_t max_len;
+    uint32_t base;     // first tile (and look-back status word) of this job
+    uint32_t num_tiles;
+};
+
+// zero every job's gap words (one workgroup per job, strided)
+template <typename J>
+__global__ void gap_batch_zero_kernel(const J *jobs, uint32_t count) {
+    for (uint32_t j = blockIdx.x; j < count; j += gridDim.x) {
+        uint32_t *g = jobs[j].gap;
+        for (uint32_t i = threadIdx.x; i < jobs[j].gap_words; i += blockDim.x) g[i] = 0u;
+    }
+}
+
+template <int THREADS, int K>
+__global__ __launch_bounds__(THREADS) void enc_batch_kernel(
+    const EncJobDev *__restrict__ jobs, uint32_t count, uint32_t total_tiles,
+    const uint32_t *__restrict__ tables, uint32_t *__restrict__ counter,
+    uint64_t *__restrict__ status, uint32_t out_words, uint32_t wpl) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    __shared__ uint32_t s_tab[256];
+    __shared__ uint32_t s_wave[THREADS / 64];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_tile, s_job;
+    uint32_t *s_out = s_dyn;
+    uint32_t *s_gap = s_dyn + out_words;
+    uint32_t loaded = 0xFFFFFFFFu;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t g = atomicAdd(counter, 1u);
+            s_tile = g;
+            if (g < total_tiles) s_job = find_job(jobs, count, g);
+        }
+        __syncthreads();
+        const uint32_t g = s_tile;
+        if (g >= total_tiles) break;
+        const uint32_t j = s_job;
+        const EncJobDev job = jobs[j];
+        if (job.table != loaded) {  // uniform: every thread sees the same job
+            for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = tables[256u * job.table + i];
+            loaded = job.table;
+            __syncthreads();
+        }
+        EncBuffer buf;
+        buf.in = job.in;
+        buf.n = job.n;
+        buf.payload = job.payload;
+        buf.payload_words = job.payload_words;
+        buf.gap_words = job.gap_words;
+        buf.gap = job.gap;
+        buf.status = status + job.base;
+        buf.num_tiles = job.num_tiles;
+        buf.reserved = 0;
+        enc_tile<THREADS, K>(buf, g - job.base, s_tab, s_out, s_gap, s_wave, &s_excl, wpl);
+    }
+}
+
+struct DecJobDev {
+    const uint32_t *payload;
+    const uint32_t *gap;
+    uint8_t *out;
+    uint64_t n;
+    const uint32_t *table;
+    uint32_t payload_words;
+    uint32_t gap_elements;
+    uint32_t prefix_bits;
+    uint32_t l1_entries;
+    uint32_t table_words;
+    uint32_t max_symbols;
+    uint32_t max_length;
+    uint32_t base;     // first tile (and look-back status word) of this job
+    uint32_t num_tiles;
+    uint32_t reserved;
+};
+
+template <int THREADS, int NDW>
+__global__ __launch_bounds__(THREADS) void dec_batch_kernel(
+    const DecJobDev *__restrict__ jobs, uint32_t count, uint32_t total_tiles,
+    uint32_t slot_offset, uint32_t *__restrict__ counter, uint64_t *__restrict__ status,
+    uint32_t wpl) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    __shared__ uint32_t s_wave[THREADS / 64];
+    __shared__ uint64_t s_base;
+    __shared__ uint32_t s_tile, s_job;
+    uint32_t *s_tab = s_dyn;
+    uint32_t *s_pay = s_dyn + slot_offset;
+    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_pay + dec_payload_words(THREADS));
+    uint32_t loaded = 0xFFFFFFFFu;
+    while (true) {
+        if (threadIdx.x == 0) {
+            const uint32_t g = atomicAdd(counter, 1u);
+            s_tile = g;
+            if (g < total_tiles) s_job = find_job(jobs, count, g);
+        }
+        __syncthreads();
+        const uint32_t g = s_tile;
+        if (g >= total_tiles) break;
+        const uint32_t j = s_job;
+        const DecJobDev job = jobs[j];
+        if (j != loaded) {  // uniform
+            for (uint32_t i = threadIdx.x; i < job.table_words; i += THREADS) s_tab[i] = job.table[i];
+            loaded = j;
+            __syncthreads();
+        }
+        DecBuffer buf;
+        buf.payload = job.payload;
+        buf.payload_words = job.payload_words;
+        buf.gap_elements = job.gap_elements;
+        buf.gap = job.gap;
+        buf.out = job.out;
+        buf.n = job.n;
+        buf.status = status + job.base;
+        buf.num_tiles = job.num_tiles;
+        buf.reserved = 0;
+        const DecTable t{job.prefix_bits, job.l1_entries,
+                         job.max_length > job.prefix_bits ? job.max_length : job.prefix_bits};
+        dec_tile<THREADS, NDW, false>(buf, g - job.base, t, s_tab, s_pay, s_bytes, s_wave,
+                                      &s_base, wpl);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Synthetic data (counter-based splitmix64 + integer inverse CDF).
 struct SynthTable {
     uint64_t cdf[256];
@@ -869,6 +1079,66 @@ static hgap_launch_config resolve(const hgap_launch_config *cfg) {
     return c;
 }
 
+
+// ---- batched launches -----------------------------------------------------------------------
+static_assert(sizeof(HistJobDev) == sizeof(hgap_hist_job), "hgap_hist_job layout");
+static_assert(sizeof(EncJobDev) == sizeof(hgap_enc_job), "hgap_enc_job layout");
+static_assert(sizeof(DecJobDev) == sizeof(hgap_dec_job), "hgap_dec_job layout");
+
+template <int THREADS, int K>
+static hgap_status launch_enc_batch_t(const EncJobDev *jobs, uint32_t count, uint32_t total,
+                                      const uint32_t *tables, uint32_t *ws, uint32_t max_len,
+                                      const hgap_launch_config &c, hipStream_t s) {
+    constexpr uint32_t tile = THREADS * K;
+    const uint32_t out_words = enc_out_words(tile, max_len);
+    const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
+    auto kern = enc_batch_kernel<THREADS, K>;
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total, tables, ws,
+                       reinterpret_cast<uint64_t *>(ws + 4), out_words, window_per_lane(c));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+static hgap_status launch_enc_batch(const EncJobDev *jobs, uint32_t count, uint32_t total,
+                                    const uint32_t *tables, uint32_t *ws, uint32_t max_len,
+                                    const hgap_launch_config &c, hipStream_t s) {
+#define HGAP_ENC_CASE(T, K)                                                                    \
+    if (c.enc_threads == T && c.enc_bytes_per_thread == K)                                   \
+        return launch_enc_batch_t<T, K>(jobs, count, total, tables, ws, max_len, c, s);
+    HGAP_ENC_CASE(256, 16) HGAP_ENC_CASE(256, 32) HGAP_ENC_CASE(256, 64)
+    HGAP_ENC_CASE(512, 16) HGAP_ENC_CASE(512, 32) HGAP_ENC_CASE(512, 64)
+    HGAP_ENC_CASE(1024, 16) HGAP_ENC_CASE(1024, 32) HGAP_ENC_CASE(1024, 64)
+#undef HGAP_ENC_CASE
+    return HGAP_ERR_UNSUPPORTED;
+}
+
+template <int THREADS, int NDW>
+static hgap_status launch_dec_batch_t(const DecJobDev *jobs, uint32_t count, uint32_t total,
+                                      uint32_t max_table_words, uint32_t *ws,
+                                      const hgap_launch_config &c, hipStream_t s) {
+    const uint32_t slot_offset = (max_table_words + 3u) & ~3u;
+    const size_t lds = 4ull * slot_offset + dec_stage_bytes(THREADS, NDW);
+    auto kern = dec_batch_kernel<THREADS, NDW>;
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total, slot_offset,
+                       ws, reinterpret_cast<uint64_t *>(ws + 4), window_per_lane(c));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+template <int THREADS>
+static hgap_status launch_dec_batch_threads(const DecJobDev *jobs, uint32_t count,
+                                            uint32_t total, uint32_t max_table_words,
+                                            uint32_t max_symbols, uint32_t *ws,
+                                            const hgap_launch_config &c, hipStream_t s) {
+    switch (dec_slot_words(max_symbols)) {
+    case 11: return launch_dec_batch_t<THREADS, 11>(jobs, count, total, max_table_words, ws, c, s);
+    case 17: return launch_dec_batch_t<THREADS, 17>(jobs, count, total, max_table_words, ws, c, s);
+    default: return launch_dec_batch_t<THREADS, 33>(jobs, count, total, max_table_words, ws, c, s);
+    }
+}
+
 }  // namespace hgap
 
 using namespace hgap;
@@ -1025,6 +1295,124 @@ hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const
         if (st != HGAP_ERR_UNSUPPORTED) break;
     }
     return st;
+}
+
+
+uint64_t hgap_batch_plan_histogram(hgap_hist_job *jobs, uint32_t count) {
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < count; j++) {
+        jobs[j].base = base;
+        base += jobs[j].n;
+    }
+    return base;
+}
+
+hgap_status hgap_batch_histogram(const hgap_hist_job *d_jobs, uint32_t count, uint64_t total,
+                                 hgap_stream_t stream) {
+    if (count == 0) return HGAP_OK;
+    if (!d_jobs) return HGAP_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    (void)hipGetLastError();
+    const HistJobDev *jobs = reinterpret_cast<const HistJobDev *>(d_jobs);
+    hipLaunchKernelGGL(hist_batch_zero_kernel, dim3(count < 4096 ? count : 4096), dim3(256), 0, s,
+                       jobs, count);
+    if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
+    if (total == 0) return HGAP_OK;
+    uint64_t want = (total + (1u << 20) - 1) >> 20;  // at least 1 MiB per workgroup
+    const unsigned blocks = grid_for(hist_batch_kernel, kHistThreads, 0, 0, want ? want : 1);
+    hipLaunchKernelGGL(hist_batch_kernel, dim3(blocks), dim3(kHistThreads), 0, s, jobs, count,
+                       total);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+hgap_status hgap_encode_table(const hgap_codebook *cb, uint32_t table[256]) {
+    if (!cb || !table) return HGAP_ERR_INVALID_ARG;
+    for (int b = 0; b < 256; b++) {
+        const uint32_t l = cb->length[b];
+        if (l > HGAP_MAX_CODE_LEN || (l > 0 && (cb->code[b] >> l) != 0)) return HGAP_ERR_CODEBOOK;
+        table[b] = (cb->code[b] << 5) | l;
+    }
+    return HGAP_OK;
+}
+
+uint32_t hgap_batch_plan_encode(hgap_enc_job *jobs, uint32_t count,
+                                const hgap_launch_config *cfg) {
+    const hgap_launch_config c = resolve(cfg);
+    const uint64_t tile = (uint64_t)c.enc_threads * c.enc_bytes_per_thread;
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < count; j++) {
+        jobs[j].base = (uint32_t)base;
+        jobs[j].num_tiles = (uint32_t)((jobs[j].n + tile - 1) / tile);
+        base += jobs[j].num_tiles;
+    }
+    return (uint32_t)base;
+}
+
+uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
+                                const hgap_launch_config *cfg) {
+    const hgap_launch_config c = resolve(cfg);
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < count; j++) {
+        jobs[j].base = (uint32_t)base;
+        jobs[j].num_tiles = jobs[j].n ? (jobs[j].gap_elements + c.dec_threads - 1) / c.dec_threads
+                                      : 0;
+        base += jobs[j].num_tiles;
+    }
+    return (uint32_t)base;
+}
+
+size_t hgap_batch_workspace_bytes(uint32_t total_tiles) { return 16 + 8ull * total_tiles; }
+
+hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32_t total_tiles,
+                              uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
+                              size_t workspace_bytes, const hgap_launch_config *cfg,
+                              hgap_stream_t stream) {
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
+    if (count == 0 || total_tiles == 0) return HGAP_OK;
+    if (!d_jobs || !d_tables || !d_workspace || !aligned16(d_workspace))
+        return HGAP_ERR_INVALID_ARG;
+    if (max_len < 1 || max_len > HGAP_MAX_CODE_LEN) return HGAP_ERR_CODEBOOK;
+    if (workspace_bytes < hgap_batch_workspace_bytes(total_tiles)) return HGAP_ERR_WORKSPACE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
+    const EncJobDev *jobs = reinterpret_cast<const EncJobDev *>(d_jobs);
+    hgap_status st = launch_zero(ws, 4 + 2ull * total_tiles, nullptr, 0, s);
+    if (st != HGAP_OK) return st;
+    hipLaunchKernelGGL(gap_batch_zero_kernel<EncJobDev>, dim3(count < 4096 ? count : 4096),
+                       dim3(256), 0, s, jobs, count);
+    if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
+    return launch_enc_batch(jobs, count, total_tiles, d_tables, ws, max_len, c, s);
+}
+
+hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32_t total_tiles,
+                              uint32_t max_table_words, uint32_t max_symbols, void *d_workspace,
+                              size_t workspace_bytes, const hgap_launch_config *cfg,
+                              hgap_stream_t stream) {
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
+    if (count == 0 || total_tiles == 0) return HGAP_OK;
+    if (!d_jobs || !d_workspace || !aligned16(d_workspace)) return HGAP_ERR_INVALID_ARG;
+    if (max_symbols < 1 || max_symbols > HGAP_SEGMENT_BITS || max_table_words < 2)
+        return HGAP_ERR_CODEBOOK;
+    if (workspace_bytes < hgap_batch_workspace_bytes(total_tiles)) return HGAP_ERR_WORKSPACE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
+    const DecJobDev *jobs = reinterpret_cast<const DecJobDev *>(d_jobs);
+    hgap_status st = launch_zero(ws, 4 + 2ull * total_tiles, nullptr, 0, s);
+    if (st != HGAP_OK) return st;
+    switch (c.dec_threads) {
+    case 256:
+        return launch_dec_batch_threads<256>(jobs, count, total_tiles, max_table_words,
+                                             max_symbols, ws, c, s);
+    case 512:
+        return launch_dec_batch_threads<512>(jobs, count, total_tiles, max_table_words,
+                                             max_symbols, ws, c, s);
+    case 1024:
+        return launch_dec_batch_threads<1024>(jobs, count, total_tiles, max_table_words,
+                                              max_symbols, ws, c, s);
+    default: return HGAP_ERR_UNSUPPORTED;
+    }
 }
 
 hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,

@@ -129,3 +129,18 @@ def fill_numpy(n, start, seed, cdf, perm, chunk=1 << 22):
         r = splitmix64_at(seed, np.arange(start + a, start + b, dtype=np.uint64)) >> np.uint64(11)
         out[a:b] = perm[np.searchsorted(cdf, r, side="right")]
     return out
+
+
+def batch_spec(count=4096, seed=4, min_bytes=4 << 10, max_bytes=64 << 20, s_lo=0.6, s_hi=2.0):
+    """Buffers of the batched workload: sizes log-uniform in [min_bytes, max_bytes], each with
+    its own Zipf exponent ~ U(s_lo, s_hi) and its own seeded rank-to-byte permutation.
+    Returns a list of dicts (n, s, seed, perm_seed); deterministic in `seed`."""
+    out = []
+    lo, hi = math.log(min_bytes), math.log(max_bytes)
+    for i in range(count):
+        u1 = (splitmix64_scalar(seed, 3 * i) >> 11) / float(ONE53)
+        u2 = (splitmix64_scalar(seed, 3 * i + 1) >> 11) / float(ONE53)
+        n = int(math.exp(lo + (hi - lo) * u1))
+        out.append(dict(n=n, s=s_lo + (s_hi - s_lo) * u2, seed=(seed << 32) + i,
+                        perm_seed=splitmix64_scalar(seed, 3 * i + 2)))
+    return out

@@ -164,6 +164,77 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
                             void *d_workspace, size_t workspace_bytes,
                             const hgap_launch_config *cfg, hgap_stream_t stream);
 
+/* ---- batched steps: many independent buffers per launch ----------------------------------- */
+/* Job descriptors live in device memory (the caller copies the arrays filled by the plan
+ * functions), so a captured graph replays the same launches.  Jobs of one call share one launch
+ * shape; a job's look-back status words are [base, base + num_tiles) of the call's workspace. */
+
+typedef struct hgap_hist_job {
+    const void *in;  /* device bytes, 16-byte aligned */
+    uint64_t n;
+    uint32_t *hist;  /* device u32[256], overwritten */
+    uint32_t reserved[2];
+    uint64_t base;   /* set by hgap_batch_plan_histogram: offset in the concatenated batch */
+} hgap_hist_job;
+
+typedef struct hgap_enc_job {
+    const void *in;  /* device bytes, 16-byte aligned */
+    uint64_t n;
+    uint32_t *payload;      /* device, payload_words words */
+    uint32_t *gap;          /* device, ceil(gap_elements / 8) words (zeroed by the call) */
+    uint32_t payload_words;
+    uint32_t gap_words;
+    uint32_t table;         /* index of this job's 256-word code table (hgap_encode_table) */
+    uint32_t max_len;       /* longest code length of its codebook */
+    uint32_t base;          /* set by hgap_batch_plan_encode */
+    uint32_t num_tiles;     /* set by hgap_batch_plan_encode */
+} hgap_enc_job;
+
+typedef struct hgap_dec_job {
+    const uint32_t *payload; /* device */
+    const uint32_t *gap;     /* device */
+    void *out;               /* device, n bytes */
+    uint64_t n;
+    const uint32_t *table;   /* device decode-table image (hgap_dectable_build) */
+    uint32_t payload_words;
+    uint32_t gap_elements;
+    uint32_t prefix_bits;    /* from the table's hgap_dectable_info */
+    uint32_t l1_entries;
+    uint32_t table_words;    /* l1_entries + l2_entries */
+    uint32_t max_symbols;    /* max_symbols_per_segment */
+    uint32_t max_length;
+    uint32_t base;           /* set by hgap_batch_plan_decode */
+    uint32_t num_tiles;      /* set by hgap_batch_plan_decode */
+    uint32_t reserved;
+} hgap_dec_job;
+
+/* Sets each job's base; returns the total byte count of the batch. */
+uint64_t hgap_batch_plan_histogram(hgap_hist_job *jobs, uint32_t count);
+hgap_status hgap_batch_histogram(const hgap_hist_job *d_jobs, uint32_t count, uint64_t total,
+                                 hgap_stream_t stream);
+
+/* Codebooks of `count` histograms (count x 256 u32) on `threads` host threads (0 = all). */
+hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
+                                      uint32_t threads);
+/* The encoder's per-byte code table ((code << 5) | length) of one codebook. */
+hgap_status hgap_encode_table(const hgap_codebook *cb, uint32_t table[256]);
+
+/* Sets base/num_tiles for the tile shape in `cfg` (NULL = default); returns total tiles. */
+uint32_t hgap_batch_plan_encode(hgap_enc_job *jobs, uint32_t count,
+                                const hgap_launch_config *cfg);
+uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
+                                const hgap_launch_config *cfg);
+size_t hgap_batch_workspace_bytes(uint32_t total_tiles);
+/* max_len / max_table_words / max_symbols: maxima over the jobs (they size LDS). */
+hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32_t total_tiles,
+                              uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
+                              size_t workspace_bytes, const hgap_launch_config *cfg,
+                              hgap_stream_t stream);
+hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32_t total_tiles,
+                              uint32_t max_table_words, uint32_t max_symbols, void *d_workspace,
+                              size_t workspace_bytes, const hgap_launch_config *cfg,
+                              hgap_stream_t stream);
+
 /* Synthetic data for tests and benchmarks: byte i = perm[rank], rank = first r with
  * (splitmix64(seed + (start+i+1)*0x9E3779B97F4A7C15) >> 11) < cdf[r]  (cdf[255] = 2^53). */
 hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,

@@ -1,0 +1,120 @@
+"""Batched encode/decode (many buffers, own codebooks, one launch per shape group, HIP graph
+replay) against the oracle and the per-buffer path.  Needs an MI355X."""
+import sys
+
+import numpy as np
+import pytest
+
+from huffman_coding_gap_arrays_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def batch_data(cuda):
+    import torch
+
+    from huffman_coding_gap_arrays_amd import batch, codec
+
+    spec = synth.batch_spec(count=60, seed=9, min_bytes=1 << 10, max_bytes=3 << 20)
+    spec += [dict(n=n, s=1.2, seed=1000 + n, perm_seed=n) for n in (1, 15, 16, 17, 4095)]
+    spec.append(dict(n=20000, s=40.0, seed=77, perm_seed=1))  # one distinct byte
+    arena = batch.Arena([d["n"] for d in spec])
+    bufs = []
+    for i, d in enumerate(spec):
+        b = arena.view(i)
+        codec.synth_fill(b, d["seed"], synth.cdf_from_pmf(synth.pmf_zipf(d["s"])),
+                         synth.permutation(d["perm_seed"]))
+        bufs.append(b)
+    torch.cuda.synchronize()
+    return batch, codec, arena, bufs
+
+
+def test_batched_histograms(batch_data):
+    batch, codec, arena, bufs = batch_data
+    h = batch.histograms(bufs)
+    for i, b in enumerate(bufs):
+        assert (h[i] == np.bincount(b.cpu().numpy(), minlength=256)).all(), i
+
+
+def test_batched_codebooks_match_single(batch_data, orc):
+    batch, codec, arena, bufs = batch_data
+    h = batch.histograms(bufs)
+    cbs = batch.codebooks(h, threads=4)
+    for i in range(len(bufs)):
+        o = orc.codebook(h[i].astype(np.uint64))
+        assert list(cbs[i].code) == list(o["code"]) and list(cbs[i].length) == list(o["length"])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batched_encode_decode_bit_exact(batch_data, orc, mixed):
+    import torch
+
+    from huffman_coding_gap_arrays_amd import tuner
+
+    batch, codec, arena, bufs = batch_data
+    configs = None
+    if mixed:  # every buffer its own launch shape
+        space_e = list(tuner.enc_space())
+        space_d = list(tuner.dec_space())
+        configs = [{**space_e[(7 * i) % len(space_e)], **{k: v for k, v in
+                   space_d[(5 * i) % len(space_d)].items() if k != "lookback_window"}}
+                   for i in range(len(bufs))]
+        # LDS limits: drop shapes that cannot hold 16-bit codes with 1024 x 64 B tiles
+        for c in configs:
+            if c["enc_threads"] * c["enc_bytes_per_thread"] > 32768:
+                c["enc_bytes_per_thread"] = 32
+    say("batch codec", "mixed" if mixed else "default")
+    bc = batch.BatchCodec(bufs, configs=configs)
+    bc.run()
+    torch.cuda.synchronize()
+    for i, b in enumerate(bufs):
+        d = b.cpu().numpy()
+        if d.size == 0:
+            continue
+        ocb = orc.codebook(orc.histogram(d))
+        pay, gap = orc.encode(d, ocb)
+        assert (bc.payload_u32(i) == pay).all(), (i, d.size)
+        assert (bc.gap_u32(i) == gap).all(), (i, d.size)
+    assert bc.mismatches() == 0
+
+
+def test_batched_graph_replays(batch_data):
+    """The captured graph re-runs encode + decode exactly, replay after replay, with eager work
+    and synchronizations in between."""
+    import torch
+
+    batch, codec, arena, bufs = batch_data
+    bc = batch.BatchCodec(bufs)
+    bc.run()
+    torch.cuda.synchronize()
+    ref_pay = bc.payload.mem.clone()
+    ref_gap = bc.gap.mem.clone()
+    g = bc.capture()
+    for r in range(3):
+        bc.out.mem.zero_()
+        bc.payload.mem.fill_(0xA5)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert bc.mismatches() == 0, r
+        for i in range(len(bufs)):
+            o, s = bc.payload.offsets[i], bc.payload.sizes[i]
+            assert torch.equal(bc.payload.mem[o:o + s], ref_pay[o:o + s]), (r, i)
+            o, s = bc.gap.offsets[i], bc.gap.sizes[i]
+            assert torch.equal(bc.gap.mem[o:o + s], ref_gap[o:o + s]), (r, i)
+        _ = (bufs[0].float().sum() + 1).item()  # eager work between replays
+
+
+def test_plain_loop_matches(batch_data):
+    import torch
+
+    batch, codec, arena, bufs = batch_data
+    outs = batch.plain_loop(bufs[:20])
+    for o, b in zip(outs, bufs[:20]):
+        assert torch.equal(o, b)

@@ -373,3 +373,38 @@ def test_python_wrappers_import_and_build_tables_on_cpu():
                                             *[__import__("torch").zeros(4, dtype=__import__("torch").int32)] * 2))
     cb2, n, sz, gap, pay = codec.parse_stream(blob)
     assert n == 0 and cb2.symbol_count == 0 and gap.size == 0 and pay.size == 0
+
+
+def test_batch_job_layout_and_plans(capi, golden_codebooks):
+    """Job descriptors have the C layout; the plan functions lay out tiles back to back."""
+    from huffman_coding_gap_arrays_amd import batch
+    from huffman_coding_gap_arrays_amd._capi import Codebook, LaunchConfig, default_config
+
+    assert (batch.HIST_JOB.itemsize, batch.ENC_JOB.itemsize, batch.DEC_JOB.itemsize) == (40, 56, 80)
+    rng = np.random.default_rng(2)
+    ns = rng.integers(0, 3 << 20, 50)
+    jobs = np.zeros(50, batch.HIST_JOB)
+    jobs["n"] = ns
+    total = capi.hgap_batch_plan_histogram(vp(jobs), 50)
+    assert total == ns.sum() and (jobs["base"] == np.concatenate([[0], np.cumsum(ns)[:-1]])).all()
+    cfg = LaunchConfig.from_dict({**default_config().as_dict(), "enc_threads": 256,
+                                  "enc_bytes_per_thread": 32, "dec_threads": 512})
+    ej = np.zeros(50, batch.ENC_JOB)
+    ej["n"] = ns
+    tiles = capi.hgap_batch_plan_encode(vp(ej), 50, ctypes.byref(cfg))
+    want = -(-ns // 8192)
+    assert tiles == want.sum() and (ej["num_tiles"] == want).all()
+    assert (ej["base"] == np.concatenate([[0], np.cumsum(want)[:-1]])).all()
+    dj = np.zeros(50, batch.DEC_JOB)
+    dj["n"] = ns
+    dj["gap_elements"] = ns // 3
+    tiles = capi.hgap_batch_plan_decode(vp(dj), 50, ctypes.byref(cfg))
+    want = np.where(ns > 0, -(-(ns // 3) // 512), 0)
+    assert tiles == want.sum() and (dj["num_tiles"] == want).all()
+    # threaded codebook builds equal the one-at-a-time ones
+    g = golden_codebooks
+    keep = [i for i in range(len(g["k"])) if int(g["hist"][i].max()) < 1 << 32]
+    h = np.ascontiguousarray(g["hist"][keep], dtype=np.uint32)
+    cbs = batch.codebooks(h, threads=8)
+    for j, i in enumerate(keep):
+        assert list(cbs[j].code) == list(g["code"][i]) and list(cbs[j].length) == list(g["length"][i])
