@@ -171,9 +171,12 @@ def main():
     if args.batch_count > 0:
         log("batched workload ...")
         del out
-        from huffman_coding_gap_arrays_amd import batch
+        from huffman_coding_gap_arrays_amd import batch, predictor
 
-        batched = batch.bench(count=args.batch_count)
+        variants = [("default", None, None)]
+        if os.path.exists(predictor.DEFAULT_MODEL):
+            variants.append(("learned", None, predictor.Predictor.load(predictor.DEFAULT_MODEL)))
+        batched = batch.bench(count=args.batch_count, variants=variants)
         batched["workload"] = (f"{args.batch_count} buffers, sizes log-uniform 4 KiB..64 MiB, "
                                "Zipf s ~ U(0.6, 2.0), own permutation and codebook each")
     cpu = None

@@ -100,16 +100,25 @@ class BatchCodec:
     """Encodes and decodes a list of device uint8 buffers in batched launches."""
 
     def __init__(self, buffers, configs=None, hists=None, device="cuda", stream=None,
-                 threads=0):
+                 threads=0, dec_configs=None, predictor=None):
+        """configs[i] / dec_configs[i]: launch shapes of buffer i's encode / decode (dicts of
+        hgap_launch_config fields; dec_configs defaults to configs).  With a `predictor`
+        (predictor.Predictor), both come from its per-buffer choice instead."""
         lib = _lib()
         self.device = device
         self.buffers = list(buffers)
         B = len(self.buffers)
-        self.configs = [dict(c) for c in (configs or [{}] * B)]
         if hists is None:
             hists = histograms(self.buffers, stream, device)
         self.hists = np.ascontiguousarray(hists, dtype=np.uint32)
         self.codebooks = codebooks(self.hists, threads)
+        if predictor is not None:
+            from .predictor import choose_configs
+
+            configs, dec_configs, _ = choose_configs(self.hists, self.codebooks, predictor,
+                                                     fallback=False)
+        self.configs = [dict(c) for c in (configs or [{}] * B)]
+        self.dec_configs = [dict(c) for c in (dec_configs or self.configs)]
         self.sizes = []
         for i in range(B):
             sz = Sizes()
@@ -129,7 +138,7 @@ class BatchCodec:
         infos, images = [], []
         for i, cb in enumerate(self.codebooks):
             info = DectableInfo()
-            p = int(self.configs[i].get("prefix_bits", 0))
+            p = int(self.dec_configs[i].get("prefix_bits", 0))
             check(lib.hgap_dectable_build(ctypes.byref(cb), p, ctypes.byref(info), None, 0),
                   "hgap_dectable_build")
             img = np.zeros(max(info.bytes // 4, 4), np.uint32)
@@ -156,7 +165,10 @@ class BatchCodec:
                 continue
             c = _cfg_struct(self.configs[i])
             enc_groups[(c.enc_threads, c.enc_bytes_per_thread, c.lookback_window)].append(i)
-            dec_groups[(c.dec_threads, c.lookback_window)].append(i)
+            d = _cfg_struct(self.dec_configs[i])
+            # one LDS slot size per launch: every member's shape then fits when it fits alone
+            slot = lib.hgap_decode_slot_words(self.dec_infos[i].max_symbols_per_segment)
+            dec_groups[(d.dec_threads, d.lookback_window, slot)].append(i)
         self.enc_launches = []
         for (t, k, w), idx in sorted(enc_groups.items()):
             cfg = _cfg_struct(dict(enc_threads=t, enc_bytes_per_thread=k, lookback_window=w))
@@ -181,7 +193,7 @@ class BatchCodec:
                                           count=len(idx), total=total, max_len=max_len, ws=ws,
                                           members=idx))
         self.dec_launches = []
-        for (t, w), idx in sorted(dec_groups.items()):
+        for (t, w, _), idx in sorted(dec_groups.items()):
             cfg = _cfg_struct(dict(dec_threads=t, lookback_window=w))
             jobs = np.zeros(len(idx), DEC_JOB)
             max_tw, max_sym = 2, 1
@@ -299,12 +311,16 @@ def make_workload(count=4096, seed=4, device="cuda", **kw):
     return arena, bufs, spec
 
 
-def bench(count=4096, seed=4, reps=3, configs=None, device="cuda"):
+def bench(count=4096, seed=4, reps=3, device="cuda", variants=(("default", None, None),)):
     """Wall-clock comparison on the batched workload (all inputs resident in HBM):
     plain   - per-buffer histogram/codebook/encode/table/decode calls, one buffer at a time;
-    batched - batched histogram, threaded codebooks, planning and table upload, batched encode
-              and decode (the whole pipeline from raw buffers);
-    graph   - replay of the captured encode + decode graph (codebooks and tables reused)."""
+    batched - batched histogram, threaded codebooks, launch-shape choice, planning and table
+              upload, batched encode and decode (the whole pipeline from raw buffers);
+    graph   - replay of the captured encode + decode graph (codebooks and tables reused; the
+              payload and output arenas are cleared first, so the check after the replays sees
+              only what they wrote).
+    `variants`: (name, configs, predictor) per batched run; the launch shapes come from
+    `configs` (None = defaults) or from the predictor's per-buffer choice."""
     import time
 
     arena, bufs, spec = make_workload(count, seed, device)
@@ -325,26 +341,33 @@ def bench(count=4096, seed=4, reps=3, configs=None, device="cuda"):
     plain_bad = sum(0 if torch.equal(o, b) else 1 for o, b in zip(plain_outs, bufs))
     del plain_outs
     t_plain = timed(lambda: plain_loop(bufs, workspace=ws))
-    holder = {}
+    out = dict(buffers=count, total_bytes=total, plain_s=round(t_plain, 5),
+               plain_GBps=round(total / t_plain / 1e9, 2), plain_mismatches=int(plain_bad))
+    for name, configs, predictor in variants:
+        holder = {}
 
-    def batched():
-        bc = BatchCodec(bufs, configs=configs, device=device)
-        bc.run()
-        holder["bc"] = bc
+        def batched():
+            bc = BatchCodec(bufs, configs=configs, device=device, predictor=predictor)
+            bc.run()
+            holder["bc"] = bc
 
-    t_batched = timed(batched)
-    bc = holder["bc"]
-    bad = bc.mismatches()
-    g = bc.capture()
-    t_graph = timed(g.replay)
-    bad_graph = bc.mismatches()
-    return dict(buffers=count, total_bytes=total, compressed_bytes=bc.compressed_bytes,
-                plain_s=round(t_plain, 5), batched_s=round(t_batched, 5),
-                graph_s=round(t_graph, 5),
-                plain_GBps=round(total / t_plain / 1e9, 2),
-                batched_GBps=round(total / t_batched / 1e9, 2),
-                graph_GBps=round(total / t_graph / 1e9, 2),
-                speedup_batched_vs_plain=round(t_plain / t_batched, 2),
-                speedup_graph_vs_plain=round(t_plain / t_graph, 2),
-                encode_launches=len(bc.enc_launches), decode_launches=len(bc.dec_launches),
-                mismatches=int(bad + bad_graph + plain_bad))
+        t_batched = timed(batched)
+        bc = holder["bc"]
+        bad = bc.mismatches()
+        g = bc.capture()
+        bc.payload.mem.zero_()
+        bc.out.mem.zero_()
+        t_graph = timed(g.replay)
+        bad_graph = bc.mismatches()
+        shapes = len({(c.get("enc_threads"), c.get("enc_bytes_per_thread"))
+                      for c in bc.configs})
+        out[name] = dict(batched_s=round(t_batched, 5), graph_s=round(t_graph, 5),
+                         batched_GBps=round(total / t_batched / 1e9, 2),
+                         graph_GBps=round(total / t_graph / 1e9, 2),
+                         speedup_batched_vs_plain=round(t_plain / t_batched, 2),
+                         speedup_graph_vs_plain=round(t_plain / t_graph, 2),
+                         encode_launches=len(bc.enc_launches),
+                         decode_launches=len(bc.dec_launches), encoder_tile_shapes=shapes,
+                         compressed_bytes=bc.compressed_bytes, mismatches=int(bad + bad_graph))
+        del g, bc, holder
+    return out

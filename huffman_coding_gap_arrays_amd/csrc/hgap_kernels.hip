@@ -1139,11 +1139,78 @@ static hgap_status launch_dec_batch_threads(const DecJobDev *jobs, uint32_t coun
     }
 }
 
+// Whether static + `dyn_bytes` of dynamic LDS fit the CU (HIP's error state left clear).
+template <typename K>
+static bool lds_fits(K kernel, size_t dyn_bytes) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kernel)) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.sharedSizeBytes + dyn_bytes <= kMaxLds;
+}
+
+template <int THREADS, int K>
+static bool enc_shape_fits(uint32_t max_len) {
+    constexpr uint32_t tile = THREADS * K;
+    const size_t lds = 4ull * (enc_out_words(tile, max_len) + enc_gap_words(tile, max_len));
+    return lds_fits(enc_kernel<THREADS, K>, lds) && lds_fits(enc_batch_kernel<THREADS, K>, lds);
+}
+
+static bool enc_fits(const hgap_launch_config &c, uint32_t max_len) {
+#define HGAP_ENC_CASE(T, K) \
+    if (c.enc_threads == T && c.enc_bytes_per_thread == K) return enc_shape_fits<T, K>(max_len);
+    HGAP_ENC_CASE(256, 16) HGAP_ENC_CASE(256, 32) HGAP_ENC_CASE(256, 64)
+    HGAP_ENC_CASE(512, 16) HGAP_ENC_CASE(512, 32) HGAP_ENC_CASE(512, 64)
+    HGAP_ENC_CASE(1024, 16) HGAP_ENC_CASE(1024, 32) HGAP_ENC_CASE(1024, 64)
+#undef HGAP_ENC_CASE
+    return false;
+}
+
+template <int THREADS, int NDW>
+static bool dec_shape_fits(uint32_t table_words) {
+    const size_t lds = 4ull * ((table_words + 3u) & ~3u) + dec_stage_bytes(THREADS, NDW);
+    return lds_fits(dec_kernel<THREADS, NDW, true>, lds) &&
+           lds_fits(dec_kernel<THREADS, NDW, false>, lds) &&
+           lds_fits(dec_batch_kernel<THREADS, NDW>, lds);
+}
+
+template <int THREADS>
+static bool dec_threads_fit(const hgap_dectable_info &info) {
+    const uint32_t tw = info.l1_entries + info.l2_entries;
+    switch (dec_slot_words(info.max_symbols_per_segment)) {
+    case 11: return dec_shape_fits<THREADS, 11>(tw);
+    case 17: return dec_shape_fits<THREADS, 17>(tw);
+    default: return dec_shape_fits<THREADS, 33>(tw);
+    }
+}
+
+static bool dec_fits(const hgap_launch_config &c, const hgap_dectable_info &info) {
+    switch (c.dec_threads) {
+    case 256: return dec_threads_fit<256>(info);
+    case 512: return dec_threads_fit<512>(info);
+    case 1024: return dec_threads_fit<1024>(info);
+    default: return false;
+    }
+}
+
 }  // namespace hgap
 
 using namespace hgap;
 
 extern "C" {
+
+hgap_status hgap_config_supported(const hgap_launch_config *cfg, uint32_t max_len,
+                                  const hgap_dectable_info *info) {
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c) || max_len > HGAP_MAX_CODE_LEN) return HGAP_ERR_INVALID_ARG;
+    if (max_len > 0 && !enc_fits(c, max_len)) return HGAP_ERR_UNSUPPORTED;
+    if (info && !dec_fits(c, *info)) return HGAP_ERR_UNSUPPORTED;
+    return HGAP_OK;
+}
+
+uint32_t hgap_decode_slot_words(uint32_t max_symbols) { return dec_slot_words(max_symbols); }
+
 
 int hgap_device_cu_count(void) { return device_cus(); }
 

@@ -235,6 +235,37 @@ hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32
                               size_t workspace_bytes, const hgap_launch_config *cfg,
                               hgap_stream_t stream);
 
+/* HGAP_OK when the launch shape `cfg` fits the device (LDS) for an encoder whose longest code
+ * is `max_len` (0 = skip the encoder check) and a decode table described by `info` (NULL = skip
+ * the decoder check), for both the single-buffer and the batched kernels; HGAP_ERR_UNSUPPORTED
+ * when it does not.  No counterpart in the reference (its launch shape is fixed). */
+hgap_status hgap_config_supported(const hgap_launch_config *cfg, uint32_t max_len,
+                                  const hgap_dectable_info *info);
+/* LDS words of one decoder lane's output slot for tables whose segments hold at most
+ * `max_symbols` codewords; batched decode launches group buffers of one slot size. */
+uint32_t hgap_decode_slot_words(uint32_t max_symbols);
+
+/* ---- launch-shape predictor network ---------------------------------------------------------
+ * A two-layer perceptron y = W2 relu(W1 x + b1) + b2 in fp32, trained by full-batch Adam on a
+ * masked mean squared error.  Parameters are one flat device array: W1 [hidden x in] row-major,
+ * b1 [hidden], W2 [out x hidden] row-major, b2 [out].  No counterpart in the reference. */
+uint32_t hgap_mlp_param_count(uint32_t in, uint32_t hidden, uint32_t out);
+/* d_x [n x in] -> d_h [n x hidden] (post-ReLU), d_y [n x out]. */
+hgap_status hgap_mlp_forward(const float *d_x, uint32_t n, uint32_t in, uint32_t hidden,
+                             uint32_t out, const float *d_params, float *d_h, float *d_y,
+                             hgap_stream_t stream);
+/* Gradient of L = sum(mask * (y - target)^2) / mask_sum with respect to the parameters, from
+ * the forward pass's d_h and d_y; d_dy [n x out] and d_dh [n x hidden] are scratch outputs. */
+hgap_status hgap_mlp_backward(const float *d_x, const float *d_h, const float *d_y,
+                              const float *d_target, const float *d_mask, uint32_t n, uint32_t in,
+                              uint32_t hidden, uint32_t out, float mask_sum,
+                              const float *d_params, float *d_dy, float *d_dh, float *d_grad,
+                              hgap_stream_t stream);
+/* One Adam update (step counts from 1) of `count` parameters with moments d_m, d_v. */
+hgap_status hgap_mlp_adam_step(float *d_params, const float *d_grad, float *d_m, float *d_v,
+                               uint32_t count, uint32_t step, float lr, float beta1, float beta2,
+                               float eps, hgap_stream_t stream);
+
 /* Synthetic data for tests and benchmarks: byte i = perm[rank], rank = first r with
  * (splitmix64(seed + (start+i+1)*0x9E3779B97F4A7C15) >> 11) < cdf[r]  (cdf[255] = 2^53). */
 hgap_status hgap_synth_fill(void *d_out, uint64_t n, uint64_t start, uint64_t seed,
