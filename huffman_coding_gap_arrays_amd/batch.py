@@ -11,7 +11,6 @@ Per-buffer launch shapes (encoder tile shape, decoder tile size, decode-table sp
 sharing a shape share a launch.
 """
 import ctypes
-from collections import defaultdict
 
 import numpy as np
 import torch
@@ -22,6 +21,8 @@ from ._capi import Codebook, DecJob, DectableInfo, EncJob, HistJob, LaunchConfig
 HIST_JOB = np.dtype(HistJob)
 ENC_JOB = np.dtype(EncJob)
 DEC_JOB = np.dtype(DecJob)
+SIZES = np.dtype(Sizes)
+INFO = np.dtype(DectableInfo)
 ALIGN = 256  # arena alignment of every buffer (bytes)
 
 
@@ -50,21 +51,24 @@ class Arena:
     """One device allocation carved into 256-byte aligned views."""
 
     def __init__(self, sizes, device="cuda"):
-        self.sizes = [int(s) for s in sizes]
-        self.offsets = []
-        off = 0
-        for s in self.sizes:
-            self.offsets.append(off)
-            off = _up(ALIGN, off + max(s, 1))
-        self.mem = torch.empty(max(off, ALIGN), dtype=torch.uint8, device=device)
+        self.sizes = np.asarray(sizes, dtype=np.int64).reshape(-1)
+        padded = _up(ALIGN, np.maximum(self.sizes, 1))
+        self.offsets = np.zeros(len(self.sizes), np.int64)
+        if len(padded) > 1:
+            self.offsets[1:] = np.cumsum(padded)[:-1]
+        total = int(padded.sum()) if len(padded) else 0
+        self.mem = torch.empty(max(total, ALIGN), dtype=torch.uint8, device=device)
         self.base = self.mem.data_ptr()
 
     def view(self, i):
-        o, s = self.offsets[i], self.sizes[i]
+        o, s = int(self.offsets[i]), int(self.sizes[i])
         return self.mem[o:o + s]
 
     def ptr(self, i):
-        return self.base + self.offsets[i]
+        return self.base + int(self.offsets[i])
+
+    def ptrs(self, idx=slice(None)):
+        return (self.offsets[idx] + self.base).astype(np.uint64)
 
 
 def histograms(buffers, stream=None, device="cuda"):
@@ -72,10 +76,9 @@ def histograms(buffers, stream=None, device="cuda"):
     B = len(buffers)
     hist = torch.empty((max(B, 1), 256), dtype=torch.int32, device=device)
     jobs = np.zeros(B, HIST_JOB)
-    for i, b in enumerate(buffers):
-        jobs[i]["in_"] = b.data_ptr()
-        jobs[i]["n"] = b.numel()
-        jobs[i]["hist"] = hist.data_ptr() + 1024 * i
+    jobs["in_"] = [b.data_ptr() for b in buffers]
+    jobs["n"] = [b.numel() for b in buffers]
+    jobs["hist"] = hist.data_ptr() + 1024 * np.arange(B, dtype=np.uint64)
     total = _lib().hgap_batch_plan_histogram(_vp(jobs), B)
     d_jobs = _to_device(jobs, device)
     check(_lib().hgap_batch_histogram(ctypes.c_void_p(d_jobs.data_ptr()), B, total,
@@ -96,6 +99,17 @@ def _cfg_struct(cfg):
     return LaunchConfig.from_dict(cfg or {})
 
 
+def _config_arrays(configs, B):
+    """Per-buffer launch-shape fields (dict of uint32 arrays) from a list of config dicts."""
+    d = _capi.default_config().as_dict()
+    out = {k: np.full(B, v, np.uint32) for k, v in d.items()}
+    if configs:
+        for i, c in enumerate(configs):
+            for k, v in c.items():
+                out[k][i] = v
+    return out
+
+
 class BatchCodec:
     """Encodes and decodes a list of device uint8 buffers in batched launches."""
 
@@ -103,122 +117,128 @@ class BatchCodec:
                  threads=0, dec_configs=None, predictor=None):
         """configs[i] / dec_configs[i]: launch shapes of buffer i's encode / decode (dicts of
         hgap_launch_config fields; dec_configs defaults to configs).  With a `predictor`
-        (predictor.Predictor), both come from its per-buffer choice instead."""
+        (predictor.Predictor), both come from its per-buffer choice instead.  Host preparation
+        (codebooks, sizes, code tables, decode tables) runs on `threads` host threads."""
+        from . import predictor as pr
+
         lib = _lib()
         self.device = device
         self.buffers = list(buffers)
-        B = len(self.buffers)
+        B = self.count = len(self.buffers)
         if hists is None:
             hists = histograms(self.buffers, stream, device)
-        self.hists = np.ascontiguousarray(hists, dtype=np.uint32)
-        self.codebooks = codebooks(self.hists, threads)
-        if predictor is not None:
-            from .predictor import choose_configs
-
-            configs, dec_configs, _ = choose_configs(self.hists, self.codebooks, predictor,
-                                                     fallback=False)
+        self.hists = np.ascontiguousarray(hists, dtype=np.uint32).reshape(B, 256)
+        self.cb_array = (Codebook * max(B, 1))()
+        self.size_array = (Sizes * max(B, 1))()
+        tables = np.zeros((max(B, 1), 256), np.uint32)
+        check(lib.hgap_batch_prepare(_vp(self.hists), B, self.cb_array, self.size_array,
+                                     _vp(tables), threads), "hgap_batch_prepare")
+        sv = np.frombuffer(self.size_array, dtype=SIZES, count=B)
+        lengths = pr.code_lengths(self.cb_array, B)
+        self.max_len = np.maximum(lengths.max(1) if B else np.zeros(0, np.uint32), 1)
+        if predictor is not None and B:
+            X = pr.features_batch(self.hists, lengths)
+            em, dm = pr.support_masks(self.cb_array, B, self.max_len, threads)
+            ei, di = pr.choose_indices(predictor, X, em, dm)
+            configs = [pr.ENC_CANDIDATES[j] for j in ei]
+            dec_configs = [pr.DEC_CANDIDATES[j] for j in di]
         self.configs = [dict(c) for c in (configs or [{}] * B)]
         self.dec_configs = [dict(c) for c in (dec_configs or self.configs)]
-        self.sizes = []
-        for i in range(B):
-            sz = Sizes()
-            check(lib.hgap_encode_sizes(ctypes.byref(self.codebooks[i]), _vp(self.hists[i]),
-                                        ctypes.byref(sz)), "hgap_encode_sizes")
-            self.sizes.append(sz)
-        ns = [b.numel() for b in self.buffers]
-        self.payload = Arena([4 * s.payload_words for s in self.sizes], device)
-        self.gap = Arena([4 * s.gap_words for s in self.sizes], device)
-        self.out = Arena(ns, device)
-        # encoder code tables, one per buffer
-        tables = np.zeros((max(B, 1), 256), np.uint32)
-        for i, cb in enumerate(self.codebooks):
-            check(lib.hgap_encode_table(ctypes.byref(cb), _vp(tables[i])), "hgap_encode_table")
+        self.enc_cfg = _config_arrays(self.configs, B)
+        self.dec_cfg = _config_arrays(self.dec_configs, B)
+        self.ns = np.array([b.numel() for b in self.buffers], np.uint64)
+        self.in_ptrs = np.array([b.data_ptr() for b in self.buffers], np.uint64)
+        self.payload_words = sv["payload_words"].copy()
+        self.gap_words = sv["gap_words"].copy()
+        self.gap_elements = sv["gap_elements"].copy()
+        self.payload = Arena(4 * self.payload_words.astype(np.int64), device)
+        self.gap = Arena(4 * self.gap_words.astype(np.int64), device)
+        self.out = Arena(self.ns.astype(np.int64), device)
         self.d_enc_tables = _to_device(tables, device)
-        # decode tables, one per buffer (first-level width from its config)
-        infos, images = [], []
-        for i, cb in enumerate(self.codebooks):
-            info = DectableInfo()
-            p = int(self.dec_configs[i].get("prefix_bits", 0))
-            check(lib.hgap_dectable_build(ctypes.byref(cb), p, ctypes.byref(info), None, 0),
-                  "hgap_dectable_build")
-            img = np.zeros(max(info.bytes // 4, 4), np.uint32)
-            if info.bytes:
-                check(lib.hgap_dectable_build(ctypes.byref(cb), p, ctypes.byref(info), _vp(img),
-                                              img.nbytes), "hgap_dectable_build")
-            infos.append(info)
-            images.append(img)
-        self.dec_infos = infos
-        self.dec_tables = Arena([im.nbytes for im in images], device)
+        # decode tables, one per buffer, first-level width from its decoder config
+        prefix = np.ascontiguousarray(self.dec_cfg["prefix_bits"])
+        infos = (DectableInfo * max(B, 1))()
+        check(lib.hgap_dectable_build_batch(self.cb_array, B, _vp(prefix), infos, None, None, 0,
+                                            threads), "hgap_dectable_build_batch")
+        iv = np.frombuffer(infos, dtype=INFO, count=B)
+        self.dec_tables = Arena(iv["bytes"].astype(np.int64), device)
         host = np.zeros(self.dec_tables.mem.numel(), np.uint8)
-        for i, im in enumerate(images):
-            o = self.dec_tables.offsets[i]
-            host[o:o + im.nbytes] = im.view(np.uint8)
+        offs = np.ascontiguousarray(self.dec_tables.offsets.astype(np.uint64))
+        check(lib.hgap_dectable_build_batch(self.cb_array, B, _vp(prefix), infos, _vp(host),
+                                            _vp(offs), host.nbytes, threads),
+              "hgap_dectable_build_batch")
+        self.dec_info = np.frombuffer(infos, dtype=INFO, count=B).copy()
         self.dec_tables.mem.copy_(torch.from_numpy(host))
         self._plan_groups()
 
+    @property
+    def codebooks(self):
+        return [self.cb_array[i] for i in range(self.count)]
+
+    @property
+    def sizes(self):
+        return [self.size_array[i] for i in range(self.count)]
+
+    def _workspace(self, total):
+        return torch.empty(_up(16, _lib().hgap_batch_workspace_bytes(total)), dtype=torch.uint8,
+                           device=self.device)
+
     def _plan_groups(self):
         lib = _lib()
-        enc_groups = defaultdict(list)
-        dec_groups = defaultdict(list)
-        for i, b in enumerate(self.buffers):
-            if b.numel() == 0:
-                continue
-            c = _cfg_struct(self.configs[i])
-            enc_groups[(c.enc_threads, c.enc_bytes_per_thread, c.lookback_window)].append(i)
-            d = _cfg_struct(self.dec_configs[i])
-            # one LDS slot size per launch: every member's shape then fits when it fits alone
-            slot = lib.hgap_decode_slot_words(self.dec_infos[i].max_symbols_per_segment)
-            dec_groups[(d.dec_threads, d.lookback_window, slot)].append(i)
+        live = np.nonzero(self.ns > 0)[0]
+        e, d, iv = self.enc_cfg, self.dec_cfg, self.dec_info
         self.enc_launches = []
-        for (t, k, w), idx in sorted(enc_groups.items()):
-            cfg = _cfg_struct(dict(enc_threads=t, enc_bytes_per_thread=k, lookback_window=w))
+        ekeys = np.stack([e["enc_threads"], e["enc_bytes_per_thread"], e["lookback_window"]], 1)
+        for key in np.unique(ekeys[live], axis=0):
+            idx = live[(ekeys[live] == key).all(1)]
+            cfg = _cfg_struct(dict(enc_threads=key[0], enc_bytes_per_thread=key[1],
+                                   lookback_window=key[2]))
             jobs = np.zeros(len(idx), ENC_JOB)
-            max_len = 1
-            for j, i in enumerate(idx):
-                sz = self.sizes[i]
-                jobs[j]["in_"] = self.buffers[i].data_ptr()
-                jobs[j]["n"] = self.buffers[i].numel()
-                jobs[j]["payload"] = self.payload.ptr(i)
-                jobs[j]["gap"] = self.gap.ptr(i)
-                jobs[j]["payload_words"] = sz.payload_words
-                jobs[j]["gap_words"] = sz.gap_words
-                jobs[j]["table"] = i
-                ml = max(self.codebooks[i].length[:]) or 1
-                jobs[j]["max_len"] = ml
-                max_len = max(max_len, ml)
+            jobs["in_"] = self.in_ptrs[idx]
+            jobs["n"] = self.ns[idx]
+            jobs["payload"] = self.payload.ptrs(idx)
+            jobs["gap"] = self.gap.ptrs(idx)
+            jobs["payload_words"] = self.payload_words[idx]
+            jobs["gap_words"] = self.gap_words[idx]
+            jobs["table"] = idx
+            jobs["max_len"] = self.max_len[idx]
             total = lib.hgap_batch_plan_encode(_vp(jobs), len(idx), ctypes.byref(cfg))
-            ws = torch.empty(_up(16, lib.hgap_batch_workspace_bytes(total)), dtype=torch.uint8,
-                             device=self.device)
             self.enc_launches.append(dict(cfg=cfg, jobs=_to_device(jobs, self.device),
-                                          count=len(idx), total=total, max_len=max_len, ws=ws,
-                                          members=idx))
+                                          count=len(idx), total=total,
+                                          max_len=int(self.max_len[idx].max()),
+                                          ws=self._workspace(total), members=idx))
         self.dec_launches = []
-        for (t, w, _), idx in sorted(dec_groups.items()):
-            cfg = _cfg_struct(dict(dec_threads=t, lookback_window=w))
+        ms = iv["max_symbols_per_segment"].astype(np.int64)
+        um = np.unique(ms[live]) if len(live) else np.zeros(0, np.int64)
+        slot = np.zeros(len(ms), np.int64)
+        if len(um):
+            slot[live] = np.array([lib.hgap_decode_slot_words(int(m)) for m in um])[
+                np.searchsorted(um, ms[live])]
+        tw = iv["l1_entries"].astype(np.int64) + iv["l2_entries"]
+        # one LDS slot size per launch: every member's shape then fits when it fits alone
+        dkeys = np.stack([d["dec_threads"], d["lookback_window"], slot], 1)
+        for key in np.unique(dkeys[live], axis=0):
+            idx = live[(dkeys[live] == key).all(1)]
+            cfg = _cfg_struct(dict(dec_threads=key[0], lookback_window=key[1]))
             jobs = np.zeros(len(idx), DEC_JOB)
-            max_tw, max_sym = 2, 1
-            for j, i in enumerate(idx):
-                sz, info = self.sizes[i], self.dec_infos[i]
-                jobs[j]["payload"] = self.payload.ptr(i)
-                jobs[j]["gap"] = self.gap.ptr(i)
-                jobs[j]["out"] = self.out.ptr(i)
-                jobs[j]["n"] = self.buffers[i].numel()
-                jobs[j]["table"] = self.dec_tables.ptr(i)
-                jobs[j]["payload_words"] = sz.payload_words
-                jobs[j]["gap_elements"] = sz.gap_elements
-                jobs[j]["prefix_bits"] = info.prefix_bits
-                jobs[j]["l1_entries"] = info.l1_entries
-                jobs[j]["table_words"] = info.l1_entries + info.l2_entries
-                jobs[j]["max_symbols"] = info.max_symbols_per_segment
-                jobs[j]["max_length"] = info.max_length
-                max_tw = max(max_tw, info.l1_entries + info.l2_entries)
-                max_sym = max(max_sym, info.max_symbols_per_segment)
+            jobs["payload"] = self.payload.ptrs(idx)
+            jobs["gap"] = self.gap.ptrs(idx)
+            jobs["out"] = self.out.ptrs(idx)
+            jobs["n"] = self.ns[idx]
+            jobs["table"] = self.dec_tables.ptrs(idx)
+            jobs["payload_words"] = self.payload_words[idx]
+            jobs["gap_elements"] = self.gap_elements[idx]
+            jobs["prefix_bits"] = iv["prefix_bits"][idx]
+            jobs["l1_entries"] = iv["l1_entries"][idx]
+            jobs["table_words"] = tw[idx]
+            jobs["max_symbols"] = ms[idx]
+            jobs["max_length"] = iv["max_length"][idx]
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(idx), ctypes.byref(cfg))
-            ws = torch.empty(_up(16, lib.hgap_batch_workspace_bytes(total)), dtype=torch.uint8,
-                             device=self.device)
             self.dec_launches.append(dict(cfg=cfg, jobs=_to_device(jobs, self.device),
-                                          count=len(idx), total=total, max_tw=max_tw,
-                                          max_sym=max_sym, ws=ws, members=idx))
+                                          count=len(idx), total=total,
+                                          max_tw=int(max(2, tw[idx].max())),
+                                          max_sym=int(max(1, ms[idx].max())),
+                                          ws=self._workspace(total), members=idx))
 
     def encode(self, stream=None):
         lib = _lib()

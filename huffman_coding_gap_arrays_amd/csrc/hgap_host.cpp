@@ -2,8 +2,9 @@
 // package-merge), output sizes, decode-table image and the stream header.
 //
 // Reference citations are file:line relative to the reference repository root.
-#include <cstdint>
+#include <algorithm>
 #include <atomic>
+#include <cstdint>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -26,7 +27,11 @@ namespace hgap {
 // read back through the chain of "newest consumed package of the level below".
 class PackageMerge {
 public:
-    PackageMerge(const uint64_t *weights, int k) : w_(weights), k_(k) {}
+    PackageMerge() = default;
+    void reset(const uint64_t *weights, int k) {
+        w_ = weights;
+        k_ = k;
+    }
 
     void run(uint8_t *len) {
         for (int i = 0; i < k_; i++) len[i] = 0;
@@ -105,17 +110,20 @@ private:
         }
     }
 
-    const uint64_t *w_;
-    int k_;
+    const uint64_t *w_ = nullptr;
+    int k_ = 0;
     int head_[kLevels];
     Package nodes_[kLevels][kCap];
 };
 
 void package_merge_lengths(const uint64_t *w, int k, uint8_t *len) {
     if (k <= 0) return;
-    PackageMerge *pm = new PackageMerge(w, k);
+    // one node arena per host thread, reused across calls (it is ~260 KiB: allocating it per
+    // call costs more than the merge itself)
+    static thread_local PackageMerge *pm = nullptr;
+    if (!pm) pm = new PackageMerge();
+    pm->reset(w, k);
     pm->run(len);
-    delete pm;
 }
 
 static inline uint32_t single_entry(uint32_t sym, uint32_t len) {
@@ -169,19 +177,18 @@ hgap_status hgap_codebook_build(const uint32_t hist[256], hgap_codebook *cb) {
     if (!hist || !cb) return HGAP_ERR_INVALID_ARG;
     std::memset(cb, 0, sizeof(*cb));
     // store_symbols (symbols.cpp:29-43) then a stable sort by count (huff.cpp:17-21,79).
+    // A stable sort by count of the byte-ordered list is the sort by (count, byte): sort keys
+    // count << 8 | byte.
+    uint64_t key[256];
+    int k = 0;
+    for (int b = 0; b < 256; b++)
+        if (hist[b] != 0) key[k++] = ((uint64_t)hist[b] << 8) | (uint64_t)b;
+    std::sort(key, key + k);
     uint8_t sym[256];
     uint64_t cnt[256];
-    int k = 0;
-    for (int b = 0; b < 256; b++) {
-        if (hist[b] == 0) continue;
-        int pos = k++;
-        while (pos > 0 && cnt[pos - 1] > hist[b]) {
-            cnt[pos] = cnt[pos - 1];
-            sym[pos] = sym[pos - 1];
-            pos--;
-        }
-        cnt[pos] = hist[b];
-        sym[pos] = (uint8_t)b;
+    for (int i = 0; i < k; i++) {
+        cnt[i] = key[i] >> 8;
+        sym[i] = (uint8_t)(key[i] & 0xFFu);
     }
     cb->symbol_count = (uint32_t)k;
     if (k == 0) return HGAP_OK;
@@ -198,19 +205,27 @@ hgap_status hgap_codebook_build(const uint32_t hist[256], hgap_codebook *cb) {
 
 // Codebooks for many histograms on a pool of host threads; each is independent and built
 // exactly like hgap_codebook_build.
-hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
-                                      uint32_t threads) {
+}  // extern "C"
+
+namespace {
+
+// Runs fn(i) for i in [0, count) on `threads` host threads (0 = all cores); returns the first
+// non-OK status any call returned.
+template <typename F>
+hgap_status parallel_for(uint32_t count, uint32_t threads, F fn) {
     if (count == 0) return HGAP_OK;
-    if (!hists || !cbs) return HGAP_ERR_INVALID_ARG;
-    uint32_t hw = std::thread::hardware_concurrency();
+    const uint32_t hw = std::thread::hardware_concurrency();
     if (threads == 0) threads = hw ? hw : 1;
     if (threads > count) threads = count;
     std::atomic<uint32_t> next{0};
     std::atomic<int> err{HGAP_OK};
     auto work = [&]() {
         for (uint32_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) {
-            const hgap_status st = hgap_codebook_build(hists + 256ull * i, &cbs[i]);
-            if (st != HGAP_OK) err.store(st);
+            const hgap_status st = fn(i);
+            if (st != HGAP_OK) {
+                int expected = HGAP_OK;
+                err.compare_exchange_strong(expected, st);
+            }
         }
     };
     std::vector<std::thread> pool;
@@ -218,6 +233,48 @@ hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hga
     work();
     for (auto &th : pool) th.join();
     return (hgap_status)err.load();
+}
+
+}  // namespace
+
+extern "C" {
+
+hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
+                                      uint32_t threads) {
+    if (count == 0) return HGAP_OK;
+    if (!hists || !cbs) return HGAP_ERR_INVALID_ARG;
+    return parallel_for(count, threads, [&](uint32_t i) {
+        return hgap_codebook_build(hists + 256ull * i, &cbs[i]);
+    });
+}
+
+hgap_status hgap_batch_prepare(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
+                               hgap_sizes *sizes, uint32_t *enc_tables, uint32_t threads) {
+    if (count == 0) return HGAP_OK;
+    if (!hists || !cbs || !sizes) return HGAP_ERR_INVALID_ARG;
+    return parallel_for(count, threads, [&](uint32_t i) {
+        const uint32_t *h = hists + 256ull * i;
+        hgap_status st = hgap_codebook_build(h, &cbs[i]);
+        if (st == HGAP_OK) st = hgap_encode_sizes(&cbs[i], h, &sizes[i]);
+        if (st == HGAP_OK && enc_tables) st = hgap_encode_table(&cbs[i], enc_tables + 256ull * i);
+        return st;
+    });
+}
+
+hgap_status hgap_dectable_build_batch(const hgap_codebook *cbs, uint32_t count,
+                                      const uint32_t *prefix_bits, hgap_dectable_info *infos,
+                                      void *arena, const uint64_t *offsets, size_t arena_bytes,
+                                      uint32_t threads) {
+    if (count == 0) return HGAP_OK;
+    if (!cbs || !infos || (arena && !offsets)) return HGAP_ERR_INVALID_ARG;
+    return parallel_for(count, threads, [&](uint32_t i) {
+        const uint32_t p = prefix_bits ? prefix_bits[i] : 0u;
+        if (!arena) return hgap_dectable_build(&cbs[i], p, &infos[i], nullptr, 0);
+        if (offsets[i] > arena_bytes || (offsets[i] & 3u)) return HGAP_ERR_INVALID_ARG;
+        return hgap_dectable_build(&cbs[i], p, &infos[i],
+                                   static_cast<uint8_t *>(arena) + offsets[i],
+                                   arena_bytes - offsets[i]);
+    });
 }
 
 hgap_status hgap_codebook_from_order(const uint8_t *order_symbol, const uint8_t *order_length,

@@ -64,18 +64,46 @@ def candidate_index(cfg, candidates):
 
 # ---- features --------------------------------------------------------------------------------
 
+def features_batch(hists, lengths):
+    """Feature rows [B, 5] (float64) from counts [B, 256] and code lengths by byte [B, 256]."""
+    h = np.atleast_2d(np.asarray(hists, dtype=np.float64))
+    ln = np.atleast_2d(np.asarray(lengths, dtype=np.float64))
+    n = h.sum(1)
+    safe_n = np.where(n > 0, n, 1.0)
+    p = h / safe_n[:, None]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        plogp = np.where(h > 0, p * np.log2(np.where(h > 0, p, 1.0)), 0.0)
+    k = (h > 0).sum(1)
+    X = np.stack([-plogp.sum(1), np.log2(safe_n), ln.max(1), (h * ln).sum(1) / safe_n,
+                  np.log2(np.maximum(k, 1))], 1)
+    X[n <= 0] = 0.0
+    return X
+
+
 def features(hist, lengths):
-    """Feature vector (float64[5]) of a buffer from its 256 counts and code lengths by byte."""
-    h = np.asarray(hist, dtype=np.float64)
-    ln = np.asarray(lengths, dtype=np.float64)
-    n = h.sum()
-    if n <= 0:
-        return np.array([0.0, 0.0, 0.0, 0.0, 0.0])
-    p = h[h > 0] / n
-    entropy = float(-(p * np.log2(p)).sum())
-    k = int((h > 0).sum())
-    return np.array([entropy, math.log2(n), float(ln.max()), float((h * ln).sum() / n),
-                     math.log2(k)])
+    """Feature vector (float64[5]) of a buffer from its 256 counts and code lengths by byte:
+    entropy (bits), log2 size, longest and mean code length, log2 symbol count."""
+    return features_batch(np.asarray(hist)[None], np.asarray(lengths)[None])[0]
+
+
+CODEBOOK_DTYPE = np.dtype(_capi.Codebook)
+INFO_DTYPE = np.dtype(DectableInfo)
+
+
+def codebook_array(codebooks):
+    """A contiguous ctypes array of hgap_codebook (copies when given a list)."""
+    if isinstance(codebooks, ctypes.Array):
+        return codebooks
+    arr = (_capi.Codebook * max(len(codebooks), 1))()
+    for i, cb in enumerate(codebooks):
+        ctypes.memmove(ctypes.byref(arr, i * ctypes.sizeof(_capi.Codebook)), ctypes.byref(cb),
+                       ctypes.sizeof(_capi.Codebook))
+    return arr
+
+
+def code_lengths(cb_arr, count):
+    """Code lengths by byte [count, 256] (uint32) of a codebook array."""
+    return np.frombuffer(cb_arr, dtype=CODEBOOK_DTYPE, count=count)["length"]
 
 
 def codebook_features(hist, cb):
@@ -274,47 +302,83 @@ class Predictor:
         return c[:, :N_ENC].argmin(1), c[:, N_ENC:].argmin(1)
 
 
-def _first_supported(lib, cost, candidates, args):
-    for j in np.argsort(cost, kind="stable"):
-        cfg = dict(candidates[j])
-        max_len, info = args(cfg)
-        if lib.hgap_config_supported(ctypes.byref(LaunchConfig.from_dict(cfg)), max_len,
-                                     None if info is None else ctypes.byref(info)) == 0:
-            return cfg
-    raise RuntimeError("no candidate launch shape fits this device")
+def _supported(lib, cfg, max_len, info):
+    return lib.hgap_config_supported(ctypes.byref(LaunchConfig.from_dict(cfg)), int(max_len),
+                                     None if info is None else ctypes.byref(info)) == 0
 
 
-def dectable_info(cb, prefix_bits):
-    info = DectableInfo()
-    check(_capi.load().hgap_dectable_build(ctypes.byref(cb), int(prefix_bits),
-                                           ctypes.byref(info), None, 0), "hgap_dectable_build")
-    return info
+def dectable_infos(cb_arr, count, prefix_bits, threads=0):
+    """Decode-table infos (structured numpy [count]) for first-level widths prefix_bits[i]."""
+    infos = (DectableInfo * max(count, 1))()
+    pb = np.ascontiguousarray(np.broadcast_to(np.asarray(prefix_bits, np.uint32), (count,)))
+    check(_capi.load().hgap_dectable_build_batch(cb_arr, count, pb.ctypes.data_as(ctypes.c_void_p),
+                                                 infos, None, None, 0, threads),
+          "hgap_dectable_build_batch")
+    return np.frombuffer(infos, dtype=INFO_DTYPE, count=count).copy()
 
 
-def choose_configs(hists, codebooks, predictor=None, buffers=None, fallback=True):
-    """Per-buffer (encoder config, decoder config) lists.  With a predictor, buffers whose
-    features lie in its training range get its choice; the others (or all, without a predictor)
-    are tuned exhaustively on the device when `fallback` and `buffers` are given, else get the
-    default shapes."""
-    X = np.array([codebook_features(h, cb) for h, cb in zip(hists, codebooks)]).reshape(-1, 5)
-    enc = [dict(DEFAULT_ENC) for _ in range(len(X))]
-    dec = [dict(DEFAULT_DEC) for _ in range(len(X))]
-    learned = np.zeros(len(X), bool)
-    if predictor is not None and len(X):
+def support_masks(cb_arr, count, max_lens, threads=0):
+    """(encoder [count, N_ENC], decoder [count, N_DEC]) boolean masks of the candidate shapes
+    the device can hold for each buffer (hgap_config_supported; one query per distinct key)."""
+    from . import tuner
+
+    lib = _capi.load()
+    ml = np.asarray(max_lens, np.int64)
+    enc = np.zeros((count, N_ENC), bool)
+    for L in np.unique(ml):
+        enc[ml == L] = [_supported(lib, c, max(int(L), 1), None) for c in ENC_CANDIDATES]
+    dec = np.zeros((count, N_DEC), bool)
+    for p in tuner.PREFIX_BITS:
+        iv = dectable_infos(cb_arr, count, p, threads)
+        ms = iv["max_symbols_per_segment"].astype(np.int64)
+        um = np.unique(ms)
+        slot = np.array([lib.hgap_decode_slot_words(int(m)) for m in um])[np.searchsorted(um, ms)]
+        tw = iv["l1_entries"].astype(np.int64) + iv["l2_entries"]
+        # the check depends on the table's words and the slot size only: one query per pair
+        keys, first, inv = np.unique(np.stack([tw, slot], 1), axis=0, return_index=True,
+                                     return_inverse=True)
+        cols = [j for j, c in enumerate(DEC_CANDIDATES) if c["prefix_bits"] == p]
+        sup = np.zeros((len(keys), len(cols)), bool)
+        for u in range(len(keys)):
+            info = DectableInfo()
+            info.l1_entries, info.l2_entries = int(keys[u, 0]), 0
+            info.max_symbols_per_segment = int(ms[first[u]])
+            sup[u] = [_supported(lib, DEC_CANDIDATES[j], 0, info) for j in cols]
+        dec[:, cols] = sup[inv.reshape(-1)]
+    return enc, dec
+
+
+def choose_indices(predictor, X, enc_mask, dec_mask):
+    """Per buffer, the supported encoder and decoder candidates of lowest predicted cost."""
+    c = predictor.costs(X)
+    ce = np.where(enc_mask, c[:, :N_ENC], np.inf)
+    cd = np.where(dec_mask, c[:, N_ENC:], np.inf)
+    if not np.isfinite(ce.min(1)).all() or not np.isfinite(cd.min(1)).all():
+        raise RuntimeError("no candidate launch shape fits this device for some buffer")
+    return ce.argmin(1), cd.argmin(1)
+
+
+def choose_configs(hists, codebooks, predictor=None, buffers=None, fallback=True, threads=0):
+    """Per-buffer (encoder config, decoder config, learned mask).  With a predictor, buffers whose
+    features lie in its training range get its choice among the shapes the device can hold; the
+    others (or all, without a predictor) are tuned exhaustively on the device when `fallback` and
+    `buffers` are given, else get the default shapes."""
+    B = len(hists)
+    cb_arr = codebook_array(codebooks)
+    enc = [dict(DEFAULT_ENC) for _ in range(B)]
+    dec = [dict(DEFAULT_DEC) for _ in range(B)]
+    learned = np.zeros(B, bool)
+    if predictor is not None and B:
+        lengths = code_lengths(cb_arr, B)
+        X = features_batch(np.asarray(hists, np.float64).reshape(B, 256), lengths)
         learned = predictor.in_range(X)
         idx = np.nonzero(learned)[0]
         if len(idx):
-            max_lens = [max(codebooks[i].length[:]) for i in idx]
-            # the decode table's first-level width depends on the choice: check each candidate
-            # against the table it would build
-            c = predictor.costs(X[idx])
-            lib = _capi.load()
+            em, dm = support_masks(cb_arr, B, lengths.max(1), threads)
+            ei, di = choose_indices(predictor, X[idx], em[idx], dm[idx])
             for r, i in enumerate(idx):
-                enc[i] = _first_supported(lib, c[r, :N_ENC], ENC_CANDIDATES,
-                                          lambda cfg: (int(max_lens[r]), None))
-                dec[i] = _first_supported(
-                    lib, c[r, N_ENC:], DEC_CANDIDATES,
-                    lambda cfg: (0, dectable_info(codebooks[i], cfg["prefix_bits"])))
+                enc[i] = dict(ENC_CANDIDATES[ei[r]])
+                dec[i] = dict(DEC_CANDIDATES[di[r]])
     if fallback and buffers is not None:
         from . import codec, tuner
 
@@ -322,9 +386,10 @@ def choose_configs(hists, codebooks, predictor=None, buffers=None, fallback=True
             b = buffers[i]
             if b.numel() == 0:
                 continue
-            sz = codec.encode_sizes(codebooks[i], hists[i])
-            enc[i], _ = tuner.tune_encode(b, codebooks[i], sz, reps=3)
-            e = codec.encode(b, cb=codebooks[i], sizes=sz, config=enc[i])
+            cb = cb_arr[i]
+            sz = codec.encode_sizes(cb, hists[i])
+            enc[i], _ = tuner.tune_encode(b, cb, sz, reps=3)
+            e = codec.encode(b, cb=cb, sizes=sz, config=enc[i])
             dec[i], _ = tuner.tune_decode(e, reps=3)
     return enc, dec, learned
 

@@ -218,6 +218,17 @@ hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hga
                                       uint32_t threads);
 /* The encoder's per-byte code table ((code << 5) | length) of one codebook. */
 hgap_status hgap_encode_table(const hgap_codebook *cb, uint32_t table[256]);
+/* Per buffer of a batch: codebook, sizes and (when enc_tables is not NULL) the encoder's code
+ * table (count x 256 u32), on `threads` host threads (0 = all cores). */
+hgap_status hgap_batch_prepare(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
+                               hgap_sizes *sizes, uint32_t *enc_tables, uint32_t threads);
+/* Decode tables of `count` codebooks, first-level widths prefix_bits[i] (NULL = defaults).
+ * With arena == NULL only the infos are filled (their `bytes` size the images); otherwise
+ * image i is written at arena + offsets[i] (4-byte aligned, within arena_bytes). */
+hgap_status hgap_dectable_build_batch(const hgap_codebook *cbs, uint32_t count,
+                                      const uint32_t *prefix_bits, hgap_dectable_info *infos,
+                                      void *arena, const uint64_t *offsets, size_t arena_bytes,
+                                      uint32_t threads);
 
 /* Sets base/num_tiles for the tile shape in `cfg` (NULL = default); returns total tiles. */
 uint32_t hgap_batch_plan_encode(hgap_enc_job *jobs, uint32_t count,

@@ -408,3 +408,54 @@ def test_batch_job_layout_and_plans(capi, golden_codebooks):
     cbs = batch.codebooks(h, threads=8)
     for j, i in enumerate(keep):
         assert list(cbs[j].code) == list(g["code"][i]) and list(cbs[j].length) == list(g["length"][i])
+
+
+def test_batch_prepare_and_table_batch_match_single_calls(capi, golden_codebooks):
+    from huffman_coding_gap_arrays_amd import _capi, batch
+
+    def vp(a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    g = golden_codebooks
+    # histograms whose streams fit the format's 32-bit sizes (the others report HGAP_ERR_SIZE)
+    keep = [i for i in range(len(g["k"]))
+            if int(g["hist"][i].max()) < 1 << 32 and int(g["bits"][i]) < 1 << 36]
+    h = np.ascontiguousarray(g["hist"][keep], dtype=np.uint32)
+    B = len(keep)
+    assert B > 300
+    cbs = (_capi.Codebook * B)()
+    sizes = (_capi.Sizes * B)()
+    tables = np.zeros((B, 256), np.uint32)
+    assert capi.hgap_batch_prepare(vp(h), B, cbs, sizes, vp(tables), 8) == 0
+    for j in range(B):
+        cb = _capi.Codebook()
+        assert capi.hgap_codebook_build(vp(h[j]), ctypes.byref(cb)) == 0
+        assert list(cb.code) == list(cbs[j].code) and list(cb.length) == list(cbs[j].length)
+        sz = _capi.Sizes()
+        assert capi.hgap_encode_sizes(ctypes.byref(cb), vp(h[j]), ctypes.byref(sz)) == 0
+        assert (sz.bits, sz.payload_words, sz.gap_words) == \
+            (sizes[j].bits, sizes[j].payload_words, sizes[j].gap_words)
+        t = np.zeros(256, np.uint32)
+        assert capi.hgap_encode_table(ctypes.byref(cb), vp(t)) == 0
+        assert (t == tables[j]).all()
+    # decode tables: infos, then images at caller-chosen offsets, equal to one-at-a-time builds
+    prefix = np.array([(0, 4, 8, 11, 12)[j % 5] for j in range(B)], np.uint32)
+    infos = (_capi.DectableInfo * B)()
+    assert capi.hgap_dectable_build_batch(cbs, B, vp(prefix), infos, None, None, 0, 8) == 0
+    nbytes = np.array([infos[j].bytes for j in range(B)], np.int64)
+    offs = np.zeros(B, np.uint64)
+    offs[1:] = np.cumsum((nbytes + 255) // 256 * 256)[:-1]
+    arena = np.zeros(int(offs[-1]) + int(nbytes[-1]) + 64, np.uint8)
+    assert capi.hgap_dectable_build_batch(cbs, B, vp(prefix), infos, vp(arena), vp(offs),
+                                          arena.nbytes, 8) == 0
+    for j in range(B):
+        info = _capi.DectableInfo()
+        img = np.zeros(max(1, int(nbytes[j]) // 4), np.uint32)
+        assert capi.hgap_dectable_build(ctypes.byref(cbs[j]), int(prefix[j]), ctypes.byref(info),
+                                        vp(img), img.nbytes) == 0
+        got = arena[int(offs[j]):int(offs[j]) + int(nbytes[j])].view(np.uint32)
+        assert (got == img[:len(got)]).all() and info.l1_entries == infos[j].l1_entries
+    # an arena too small is reported, not overrun
+    assert capi.hgap_dectable_build_batch(cbs, B, vp(prefix), infos, vp(arena), vp(offs),
+                                          int(offs[-1]), 8) == 7
+    assert batch.INFO.itemsize == ctypes.sizeof(_capi.DectableInfo)
