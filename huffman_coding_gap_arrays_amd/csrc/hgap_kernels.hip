@@ -425,11 +425,14 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
 // 128*s + (gap element s-1) and decodes every codeword that starts inside it (and inside the
 // payload), as in decoder.cu:505-569.  Per tile:
 //   1. each lane decodes its segment once with a 64-bit bit buffer and the two-level table in
-//      LDS, writing symbols into its own LDS slot (slots are an odd number of words apart so
-//      neighbouring lanes hit different banks); first-level entries may yield two codewords;
+//      LDS, two lookups per refill in one loop, writing symbols into its own LDS slot (slots
+//      are an odd number of words apart so neighbouring lanes hit different banks);
+//      first-level entries may yield two codewords;
 //   2. workgroup scan of the symbol counts, decoupled look-back for the tile's output offset O;
 //   3. lanes pull their slot into registers, then (after a barrier) write it compacted into the
-//      same LDS space at (O mod 16) + prefix, so the LDS image is aligned like the output;
+//      same LDS space at (O mod 16) + prefix, so the LDS image is aligned like the output:
+//      whole words with plain stores, the two words shared with neighbouring lanes zeroed
+//      first and OR-ed (byte selects by v_perm_b32);
 //   4. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
 //      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
 struct DecBuffer {
@@ -447,7 +450,6 @@ struct DecBuffer {
 struct DecTable {
     uint32_t prefix_bits;
     uint32_t l1_entries;
-    uint32_t max_step;  // most bits one lookup can consume: max(prefix_bits, longest code)
 };
 
 template <bool SINGLE>
@@ -462,17 +464,13 @@ __device__ __forceinline__ uint32_t dec_lookup(const uint32_t *s_tab, uint32_t h
     return ent;
 }
 
-// One decode step: look up the next first-level index; returns the table entry.
-#define HGAP_DEC_STEP(ENT)                                                                   \
-    const uint32_t ENT = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);
-
 template <int THREADS, int NDW, bool SINGLE>
 __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, DecTable t,
                                          const uint32_t *s_tab, uint32_t *s_pay,
                                          uint8_t *s_bytes, uint32_t *s_wave, uint64_t *s_base,
                                          uint32_t wpl) {
     const uint32_t P = t.prefix_bits;
-    // 0. stage the tile's payload words (THREADS segments x 4 words + the word after them)
+    // 0. stage the tile's payload words (THREADS segments x 4 words + the words after them)
     const uint64_t w_base = 4ull * tile * THREADS;
     {
         const uint64_t wi = w_base + 4ull * threadIdx.x;
@@ -491,74 +489,55 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     }
     __syncthreads();
 
-    // 1. decode this lane's segment into its slot
+    // 1. decode this lane's segment into its slot.  Bit positions are relative to the
+    // segment's first payload word; `bb` holds bits [p, fill) MSB-aligned, `nx` the 32 bits
+    // after `fill`.  Every step starts a codeword at p < limit; a two-codeword entry keeps its
+    // second codeword only when that one also starts before `limit`.  A step consumes at most
+    // 16 bits, so after a refill (>= 32 bits buffered) two steps can run before the next.
     const uint32_t seg = tile * THREADS + threadIdx.x;
     uint32_t cnt = 0;
     uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
     if (seg < buf.gap_elements) {
-        uint32_t start = 0;
-        if (seg > 0) start = (buf.gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
+        uint32_t p = 0;
+        if (seg > 0) p = (buf.gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
         const uint64_t avail = 32ull * buf.payload_words - 128ull * seg;
         const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
-        // bit buffer: the next q bits of the segment, MSB-aligned; `nx` = the following word
-        uint32_t k = 4 * threadIdx.x;
-        uint64_t bb = ((((uint64_t)s_pay[k]) << 32) | s_pay[k + 1]) << start;
-        int q = 64 - (int)start;
-        uint32_t nx = s_pay[k + 2];
-        k += 3;
-        uint32_t p = start;
-#define HGAP_DEC_REFILL()                                                                     \
-    if (q < 32) {                                                                             \
-        bb |= ((uint64_t)nx) << (32 - q);                                                     \
-        q += 32;                                                                              \
-        nx = s_pay[k++];                                                                      \
-    }
-#define HGAP_DEC_PAIR_STEP()                                                                  \
+        const uint32_t *pw = s_pay + 4 * threadIdx.x;
+        uint64_t bb = ((((uint64_t)pw[0]) << 32) | pw[1]) << p;
+        uint32_t fill = 64;
+        uint32_t nx = pw[2];
+#define HGAP_DEC_STEP()                                                                       \
     {                                                                                         \
-        HGAP_DEC_STEP(ent)                                                                    \
-        const uint32_t adv = (ent >> 8) & 31u;                                                \
+        const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);  \
+        const uint32_t len1 = (ent >> 24) & 31u;                                              \
+        const bool two = ((ent >> 13) & 1u) && p + len1 < limit;                              \
         slot[cnt] = (uint8_t)ent;                                                             \
         slot[cnt + 1] = (uint8_t)(ent >> 16);                                                 \
-        cnt += 1u + ((ent >> 13) & 1u);                                                       \
+        cnt += two ? 2u : 1u;                                                                 \
+        const uint32_t adv = two ? ((ent >> 8) & 31u) : len1;                                 \
         p += adv;                                                                             \
         bb <<= adv;                                                                           \
-        q -= (int)adv;                                                                        \
     }
-        // entries may hold two codewords; both start before `limit` while p + P <= limit.
-        // Two steps per refill: the second step starts at most max_step bits later, so both
-        // of its codewords start before `limit` while p + max_step + P <= limit; a step
-        // consumes at most 16 bits, so one refill every two steps keeps >= 16 bits buffered.
-        while (p + t.max_step + P <= limit) {
-            HGAP_DEC_PAIR_STEP()
-            HGAP_DEC_PAIR_STEP()
-            HGAP_DEC_REFILL()
-        }
-        while (p + P <= limit) {
-            HGAP_DEC_PAIR_STEP()
-            HGAP_DEC_REFILL()
-        }
-        // last few codewords of the segment, one at a time
         while (p < limit) {
-            HGAP_DEC_STEP(ent)
-            const uint32_t adv = (ent >> 24) & 31u;
-            slot[cnt++] = (uint8_t)ent;
-            p += adv;
-            bb <<= adv;
-            q -= (int)adv;
-            HGAP_DEC_REFILL()
+            HGAP_DEC_STEP()
+            if (p < limit) HGAP_DEC_STEP()
+            if (fill - p < 32u) {
+                bb |= ((uint64_t)nx) << (p + 32u - fill);
+                fill += 32u;
+                nx = pw[fill >> 5];
+            }
         }
-#undef HGAP_DEC_PAIR_STEP
-#undef HGAP_DEC_REFILL
+#undef HGAP_DEC_STEP
     }
 
     // 2. scan + look-back
     uint32_t tile_syms;
     const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
     // pull the slot into registers while wave 0 looks back
-    uint32_t R[NDW];
+    uint32_t R[NDW + 1];
     const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
 #pragma unroll
-    for (int j = 0; j < NDW; j++) R[j] = 0u;
+    for (int j = 0; j <= NDW; j++) R[j] = 0u;
 #pragma unroll
     for (int j = 0; j < NDW; j++) {
         if (__ballot(4u * j < cnt) == 0) break;
@@ -571,30 +550,35 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     __syncthreads();
     const uint64_t O = *s_base;
 
-    // 3. compact into LDS, aligned like the output: whole destination words with one store,
-    // the (at most two) words shared with the neighbouring lanes byte by byte
+    // 3. compact into LDS, aligned like the output (byte a of the image = output byte
+    // (O & ~15) + a).  A lane's bytes are [a, end); its words strictly inside are written
+    // whole, its first and last words (shared with neighbours) are zeroed first and OR-ed.
     const uint32_t a = (uint32_t)(O & 15u) + excl;
     const uint32_t end = a + cnt;
-    const uint32_t m = a & 3u;
-    const uint32_t d0 = a >> 2;
     uint32_t *s_words = reinterpret_cast<uint32_t *>(s_bytes);
+    uint32_t *dst = s_words + (a >> 2);
+    const uint32_t jl = cnt ? ((end - 1) >> 2) - (a >> 2) : 0u;  // index of the last word
+    if (cnt) {
+        dst[0] = 0u;
+        dst[jl] = 0u;
+    }
+    __syncthreads();
+    if (cnt) {
+        const uint32_t m = a & 3u;
+        // output word j = lane bytes [4j - m, 4j - m + 4): bytes (4 - m)..(7 - m) of {R[j]:R[j-1]}
+        const uint32_t sel = 0x07060504u - 0x01010101u * m;
+        const uint32_t himask = 0xFFFFFFFFu >> (8u * (3u - ((end - 1) & 3u)));
+        const uint32_t lomask = 0xFFFFFFFFu << (8u * m);
+        const uint32_t v0 = __builtin_amdgcn_perm(R[0], 0u, sel);
+        atomicOr(&dst[0], v0 & lomask & (jl == 0 ? himask : 0xFFFFFFFFu));
 #pragma unroll
-    for (int j = 0; j < NDW + 1; j++) {
-        const uint32_t d = d0 + j;
-        if (__ballot(cnt > 0 && 4 * d < end) == 0) break;
-        const uint32_t cur = j < NDW ? R[j] : 0u;
-        const uint32_t prev = j > 0 ? R[j - 1] : 0u;
-        const uint32_t v = m ? __builtin_amdgcn_alignbyte(cur, prev, 4u - m) : cur;
-        const uint32_t lo = 4 * d > a ? 4 * d : a;
-        const uint32_t hi = 4 * d + 4 < end ? 4 * d + 4 : end;
-        if (lo < hi) {
-            if (lo == 4 * d && hi == 4 * d + 4) {
-                s_words[d] = v;
-            } else {
-#pragma unroll
-                for (uint32_t b = 0; b < 4; b++)
-                    if (4 * d + b >= lo && 4 * d + b < hi) s_bytes[4 * d + b] = (uint8_t)(v >> (8 * b));
-            }
+        for (int j = 1; j <= NDW; j++) {
+            if (__ballot((uint32_t)j <= jl) == 0) break;
+            const uint32_t v = __builtin_amdgcn_perm(R[j], R[j - 1], sel);
+            if ((uint32_t)j < jl)
+                dst[j] = v;
+            else if ((uint32_t)j == jl)
+                atomicOr(&dst[j], v & himask);
         }
     }
     __syncthreads();
@@ -619,7 +603,6 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     }
     __syncthreads();
 }
-#undef HGAP_DEC_STEP
 
 // LDS words of the tile's staged payload: 4 per segment, plus the words a lane's bit buffer may
 // prefetch past its last segment (16-byte multiple)
@@ -860,8 +843,7 @@ __global__ __launch_bounds__(THREADS) void dec_batch_kernel(
         buf.status = status + job.base;
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
-        const DecTable t{job.prefix_bits, job.l1_entries,
-                         job.max_length > job.prefix_bits ? job.max_length : job.prefix_bits};
+        const DecTable t{job.prefix_bits, job.l1_entries};
         dec_tile<THREADS, NDW, false>(buf, g - job.base, t, s_tab, s_pay, s_bytes, s_wave,
                                       &s_base, wpl);
     }
@@ -1027,8 +1009,7 @@ static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
     auto kern = dec_kernel<THREADS, NDW, SINGLE>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
-    DecTable t{info.prefix_bits, info.l1_entries,
-               info.max_length > info.prefix_bits ? info.max_length : info.prefix_bits};
+    DecTable t{info.prefix_bits, info.l1_entries};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
                        slot_offset, counter, window_per_lane(c));
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;

@@ -21,6 +21,7 @@
  *   orc_decode_segments   round trip to the original input                     pinned
  *   orc_stream_write/read byte layout checked field by field                     pinned
  *   orc_synth_fill        numpy restatement of the counter-based generator       pinned
+ *   orc_*_mt              word-for-word equal to the serial functions            pinned
  */
 #ifndef HGAP_ORACLE_H
 #define HGAP_ORACLE_H
@@ -75,6 +76,17 @@ int64_t orc_stream_write(uint8_t *dst, uint64_t cap, int k, const uint8_t *order
 int orc_stream_parse(const uint8_t *src, uint64_t size, int *k, uint8_t *order_sym,
                      uint8_t *order_len, uint32_t *n, uint32_t *payload_words,
                      uint32_t *gap_elements, uint64_t *gap_offset, uint64_t *payload_offset);
+
+/* All-cores (OpenMP) variants with outputs identical to the serial functions above
+ * (hgap_oracle_mt.c); threads <= 0 means every available core. */
+void orc_histogram_mt(const uint8_t *in, uint64_t n, uint64_t hist[256], int threads);
+int orc_encode_mt(const uint8_t *in, uint64_t n, const uint32_t code_by_byte[256],
+                  const uint32_t len_by_byte[256], uint32_t *payload, uint64_t payload_words,
+                  uint32_t *gap, uint64_t gap_elements, int threads);
+int64_t orc_decode_segments_mt(const uint32_t *payload, uint64_t payload_words,
+                               const uint32_t *gap, uint64_t gap_elements,
+                               const uint8_t *order_sym, const uint8_t *order_len, int k,
+                               uint8_t *out, uint64_t n, int threads);
 
 uint64_t orc_splitmix64_at(uint64_t seed, uint64_t i);
 void orc_synth_fill(uint8_t *out, uint64_t n, uint64_t start, uint64_t seed,

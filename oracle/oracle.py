@@ -50,6 +50,13 @@ def load():
     L.orc_stream_write.argtypes = [_u8, ctypes.c_uint64, ctypes.c_int, _u8, _u8, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_uint32, _u32, _u32]
     L.orc_stream_write.restype = ctypes.c_int64
+    L.orc_histogram_mt.argtypes = [_u8, ctypes.c_uint64, _u64, ctypes.c_int]
+    L.orc_encode_mt.argtypes = [_u8, ctypes.c_uint64, _u32, _u32, _u32, ctypes.c_uint64, _u32,
+                                ctypes.c_uint64, ctypes.c_int]
+    L.orc_encode_mt.restype = ctypes.c_int
+    L.orc_decode_segments_mt.argtypes = [_u32, ctypes.c_uint64, _u32, ctypes.c_uint64, _u8, _u8,
+                                         ctypes.c_int, _u8, ctypes.c_uint64, ctypes.c_int]
+    L.orc_decode_segments_mt.restype = ctypes.c_int64
     L.orc_splitmix64_at.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L.orc_splitmix64_at.restype = ctypes.c_uint64
     L.orc_synth_fill.argtypes = [_u8, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u64,
@@ -63,6 +70,42 @@ def histogram(data):
     h = np.zeros(256, np.uint64)
     load().orc_histogram(d, d.size, h)
     return h
+
+
+def histogram_mt(data, threads=0):
+    d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    h = np.zeros(256, np.uint64)
+    load().orc_histogram_mt(d, d.size, h, threads)
+    return h
+
+
+def encode_mt(data, cb, sz, threads=0):
+    """orc_encode_mt: same outputs as encode(), on `threads` cores (0 = all)."""
+    d = np.ascontiguousarray(data, dtype=np.uint8).reshape(-1)
+    W, G = sz["payload_words"], sz["gap_elements"]
+    pay = np.empty(max(W, 1), np.uint32)
+    gap = np.empty(max((G + 7) // 8, 1), np.uint32)
+    rc = load().orc_encode_mt(d, d.size, cb["code"], cb["length"], pay, W, gap, G, threads)
+    if rc != 0:
+        raise ValueError(f"oracle encode_mt failed ({rc})")
+    return pay[:W], gap[: (G + 7) // 8]
+
+
+def decode_mt(payload, gap, payload_words, gap_elements, order_symbol, order_length, n,
+              threads=0):
+    pay = np.ascontiguousarray(payload, dtype=np.uint32)
+    g = np.ascontiguousarray(gap, dtype=np.uint32)
+    if pay.size == 0:
+        pay = np.zeros(1, np.uint32)
+    if g.size == 0:
+        g = np.zeros(1, np.uint32)
+    out = np.empty(max(n, 1), np.uint8)
+    produced = load().orc_decode_segments_mt(
+        pay, payload_words, g, gap_elements, np.ascontiguousarray(order_symbol, dtype=np.uint8),
+        np.ascontiguousarray(order_length, dtype=np.uint8), len(order_symbol), out, n, threads)
+    if produced < 0:
+        raise ValueError(f"oracle decode_mt failed ({produced})")
+    return out[:n]
 
 
 def codebook(hist):

@@ -101,10 +101,12 @@ def test_codebook_from_order_rejects_invalid(capi):
     assert build([1, 2, 3, 4], [1, 1, 2, 2]) == 3   # over-subscribed
 
 
-def walk_native_table(image, info, payload, gap, W, G, n):
+def walk_native_table(image, info, payload, gap, W, G, n, check_second=True):
     """Decodes with a libhgap table image following the kernel's rules in Python: segment starts
-    from the gap array; two-codeword entries while the whole first-level index lies before the
-    segment limit, single codewords after; two-level lookup for long codes."""
+    from the gap array; every step starts a codeword before the segment limit and keeps the
+    second codeword of a two-codeword entry only when it also starts before the limit;
+    two-level lookup for long codes.  check_second=False plants the mistake of keeping it
+    always."""
     P = info.prefix_bits
     bits = np.unpackbits(payload.astype(">u4").view(np.uint8))
     bits = np.concatenate([bits, np.zeros(64, np.uint8)])
@@ -117,27 +119,19 @@ def walk_native_table(image, info, payload, gap, W, G, n):
             e = int(image[info.l1_entries + (e & 0xFFFFFF) + (((win << P) & 0xFFFFFFFF) >> (32 - width))])
         return e
 
-    def pair_step(p):
-        e = lookup(p)
-        out.append(e & 0xFF)
-        if (e >> 13) & 1:
-            out.append((e >> 16) & 0xFF)
-        return p + ((e >> 8) & 31)
-
-    max_step = max(P, info.max_length)
     out = []
     for s in range(G):
         start = 0 if s == 0 else (int(gap[(s - 1) // 8]) >> (4 * ((s - 1) % 8))) & 15
         p = 128 * s + start
         end = min(128 * (s + 1), 32 * W)
-        while p + max_step + P <= end:  # two steps per iteration
-            p = pair_step(pair_step(p))
-        while p + P <= end:
-            p = pair_step(p)
         while p < end:
             e = lookup(p)
+            len1 = (e >> 24) & 31
+            two = bool((e >> 13) & 1) and (p + len1 < end or not check_second)
             out.append(e & 0xFF)
-            p += (e >> 24) & 31
+            if two:
+                out.append((e >> 16) & 0xFF)
+            p += ((e >> 8) & 31) if two else len1
     return np.array(out[:n], np.uint8)
 
 
@@ -174,6 +168,11 @@ def test_dectable_decodes_oracle_streams(capi, orc, prefix, dist):
     out = walk_native_table(img, info, pay, gap, sz["payload_words"], sz["gap_elements"], d.size)
     assert (out == d).all()
     assert info.max_symbols_per_segment == -(-128 // info.min_length)
+    if dist == "zipf" and prefix in (0, 11):
+        # keeping a second codeword that starts past the segment limit duplicates symbols
+        bad = walk_native_table(img, info, pay, gap, sz["payload_words"], sz["gap_elements"],
+                                d.size, check_second=False)
+        assert not (bad == d).all()
 
 
 def test_stream_header_round_trip_and_layout(capi, orc):

@@ -237,3 +237,23 @@ def test_synth_distribution_shape():
     expect[perm] = p * d.size
     top = np.argsort(-expect)[:20]
     assert np.all(np.abs(h[top] - expect[top]) < 6 * np.sqrt(expect[top]) + 5)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+@pytest.mark.parametrize("n", [0, 1, 17, 4096, 100_003, 2_000_001])
+def test_multithreaded_oracle_equals_serial(orc, threads, n):
+    """The all-cores variants (the multi-threaded CPU baseline) give the serial outputs word for
+    word, whatever the thread count and however the chunk edges fall."""
+    d = synth.fill_numpy(n, 0, 12, synth.cdf_from_pmf(synth.pmf_zipf(1.3)), synth.permutation(8))
+    h = orc.histogram(d)
+    assert (orc.histogram_mt(d, threads) == h).all()
+    if n == 0:
+        return
+    cb = orc.codebook(h)
+    sz = orc.sizes(h, cb["length"])
+    pay, gap = orc.encode(d, cb, sz)
+    pay2, gap2 = orc.encode_mt(d, cb, sz, threads)
+    assert (pay == pay2).all() and (gap == gap2).all()
+    out = orc.decode_mt(pay, gap, sz["payload_words"], sz["gap_elements"], cb["order_symbol"],
+                        cb["order_length"], n, threads)
+    assert (out == d).all()
