@@ -28,30 +28,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(d_dev, sample_bytes, repeats):
-    """The oracle (single-thread C restatement of the reference path) on the first
-    `sample_bytes` of the same buffer: histogram + codebook + encode + decode per repeat."""
+def cpu_baseline(d_dev, sample_bytes, repeats, threads):
+    """The oracle (C restatement of the reference path) on the first `sample_bytes` of the same
+    buffer: histogram + codebook + encode + decode per repeat, on `threads` cores (the OpenMP
+    variants) and on one core (the serial functions)."""
     from oracle import oracle
 
     oracle.load()
     sample = d_dev[:sample_bytes].cpu().numpy()
-    times = []
-    for _ in range(repeats):
+
+    def run(mt):
         t0 = time.perf_counter()
-        h = oracle.histogram(sample)
+        h = oracle.histogram_mt(sample, threads) if mt else oracle.histogram(sample)
         cb = oracle.codebook(h)
         sz = oracle.sizes(h, cb["length"])
-        pay, gap = oracle.encode(sample, cb, sz)
-        out = oracle.decode(pay, gap, sz["payload_words"], sz["gap_elements"], cb["order_symbol"],
-                            cb["order_length"], sample.size)
-        times.append(time.perf_counter() - t0)
+        if mt:
+            pay, gap = oracle.encode_mt(sample, cb, sz, threads)
+            out = oracle.decode_mt(pay, gap, sz["payload_words"], sz["gap_elements"],
+                                   cb["order_symbol"], cb["order_length"], sample.size, threads)
+        else:
+            pay, gap = oracle.encode(sample, cb, sz)
+            out = oracle.decode(pay, gap, sz["payload_words"], sz["gap_elements"],
+                                cb["order_symbol"], cb["order_length"], sample.size)
+        t = time.perf_counter() - t0
         if not (out == sample).all():
             raise RuntimeError("oracle round trip failed")
-    t = statistics.median(times)
-    return dict(value=round(sample.size / t / 1e9, 4), unit="GB/s", cores=1, kind="port",
-                sample=f"first {sample.size >> 20} MiB of the same buffer; histogram+codebook+"
-                       f"encode+decode, single thread, median of {repeats}",
-                seconds=round(t, 3))
+        return t
+
+    run(True)  # warm-up (page faults of the output buffers, thread pool start)
+    t_mt = statistics.median(run(True) for _ in range(repeats))
+    t_1 = run(False)
+    what = "histogram+codebook+encode+decode"
+    return dict(value=round(sample.size / t_mt / 1e9, 4), unit="GB/s", cores=threads,
+                kind="port",
+                sample=f"first {sample.size >> 20} MiB of the same buffer; {what} with the "
+                       f"oracle's OpenMP variants on {threads} threads, median of {repeats}",
+                seconds=round(t_mt, 4),
+                single_thread=dict(value=round(sample.size / t_1 / 1e9, 4), unit="GB/s",
+                                   cores=1, seconds=round(t_1, 3)),
+                host_cpus=os.cpu_count())
 
 
 def load_traffic(config, kernel):
@@ -70,7 +85,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="zipf_1GiB")
     ap.add_argument("--cpu-sample-mib", type=int, default=256)
-    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the CPU baseline (the GPU box's CPU share is 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096,
                     help="buffers of the batched workload (0 = skip it)")
@@ -182,7 +199,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(d_in, min(n, args.cpu_sample_mib << 20), args.cpu_repeats)
+        cpu = cpu_baseline(d_in, min(n, args.cpu_sample_mib << 20), args.cpu_repeats,
+                           args.cpu_threads)
     value = n / (ms_per_step / 1e3) / 1e9
     line = {
         "metric": "encode/decode GB/s vs 8 TB/s HBM roofline (uncompressed bytes per second of "

@@ -155,6 +155,15 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregat
 
 // ---------------------------------------------------------------------------------------------
 // zero fill (used for the histogram, the gap array and the look-back workspace)
+// Where a tile claims the workgroup's next tile: after its own look-back, so tiles are claimed
+// in the order they finish looking back (a tile never waits on a predecessor that has not
+// started), and the next tile's loads overlap this tile's compaction and stores.
+struct NextTile {
+    uint32_t *counter;  // ticket counter; nullptr = the caller claims tiles itself
+    uint32_t *s_ticket; // LDS word broadcasting the ticket
+    uint32_t tile;      // out: the claimed ticket
+};
+
 __global__ void fill_zero_kernel(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
@@ -219,21 +228,23 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
 //
 // Tile = THREADS threads x K consecutive input bytes.  Per tile:
 //   1. each thread looks up its K codes (code << 5 | length) in an LDS copy of the codebook
-//      and sums the lengths;
+//      and sums the lengths (its input bytes were loaded into registers while the previous
+//      tile was processed);
 //   2. workgroup scan of the bit counts, then decoupled look-back for the tile's global bit
 //      offset E;
 //   3. each thread packs its codes (two at a time) MSB-first into 32-bit words of a zeroed LDS
 //      tile image whose word 0 is global word (E/32) rounded down to a multiple of 4, with LDS
 //      atomic OR (a thread's first and last words may be shared with its neighbours).  When a
-//      completed word closes a 128-bit segment and a code spills past it, the spill (< 16) is
-//      that segment's gap nibble (encoder.cu:307-312), OR-ed into an LDS copy of the gap
-//      words;
+//      completed word closes a 128-bit segment, the bits of the straddling code past the
+//      boundary (< 16) are that segment's gap nibble (encoder.cu:307-312), stored as one byte
+//      per segment in LDS by the thread that completes the word;
 //   4. the tile owns global words [E/32, (E+T)/32) (the last tile also the final partial word);
 //      its first word also holds the last E%32 bits of the previous tile, which are rebuilt
 //      here from the previous tile's last input bytes instead of waiting for that tile;
 //      owned words are written with 128-bit stores;
-//   5. gap words whose eight segment boundaries all lie inside the tile are stored; the (at
-//      most two) gap words shared with neighbouring tiles are OR-ed into the pre-zeroed array.
+//   5. spill bytes are packed eight to a gap word (nibble i = segment 8g + i); gap words whose
+//      eight segment boundaries all lie inside the tile are stored; the (at most two) gap words
+//      shared with neighbouring tiles are OR-ed into the pre-zeroed array.
 struct EncTable {
     uint32_t e[256];  // (code << 5) | length, by byte value
 };
@@ -256,41 +267,73 @@ __host__ __device__ constexpr uint32_t enc_out_words(uint32_t tile_bytes, uint32
     return (tile_bytes * max_len + 31) / 32 + 8;
 }
 __host__ __device__ constexpr uint32_t enc_gap_words(uint32_t tile_bytes, uint32_t max_len) {
-    return (tile_bytes * max_len) / 1024 + 3;
+    // one spill byte per segment the tile's bits touch, offset by up to 7 bytes
+    return (tile_bytes * max_len) / 512 + 8;
+}
+
+// A thread's K input bytes of a tile, loaded into registers one tile ahead so their HBM latency
+// overlaps the previous tile's work; nb = how many of them lie inside the buffer.
+template <int K>
+struct EncStage {
+    u32x4 q[K / 16];
+    int nb;
+};
+
+template <int THREADS, int K>
+__device__ __forceinline__ EncStage<K> enc_fetch(const EncBuffer &buf, uint32_t tile) {
+    EncStage<K> st;
+#pragma unroll
+    for (int v = 0; v < K / 16; v++) st.q[v] = u32x4{0u, 0u, 0u, 0u};
+    st.nb = 0;
+    if (tile >= buf.num_tiles) return st;
+    const uint64_t mine = (uint64_t)tile * (THREADS * K) + (uint64_t)threadIdx.x * K;
+    if (mine >= buf.n) return st;
+    st.nb = buf.n - mine >= (uint64_t)K ? K : (int)(buf.n - mine);
+    if (st.nb == K) {
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(buf.in + mine);
+#pragma unroll
+        for (int v = 0; v < K / 16; v++) st.q[v] = __builtin_nontemporal_load(&p[v]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if (j < st.nb)
+                st.q[j >> 4][(j >> 2) & 3] |= (uint32_t)buf.in[mine + j] << (8 * (j & 3));
+    }
+    return st;
+}
+
+// byte j of a thread's staged input
+template <int K>
+__device__ __forceinline__ uint32_t enc_byte(const EncStage<K> &st, int j) {
+    return (st.q[j >> 4][(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
 }
 
 template <int THREADS, int K>
 __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
-                                         const uint32_t *s_tab, uint32_t *s_out,
-                                         uint32_t *s_gap, uint32_t *s_wave, uint64_t *s_excl,
-                                         uint32_t wpl) {
+                                         const EncStage<K> &st, const uint32_t *s_tab,
+                                         uint32_t *s_out, uint32_t *s_gap, uint32_t *s_wave,
+                                         uint64_t *s_excl, uint32_t wpl, NextTile &next,
+                                         EncStage<K> &next_stage) {
     constexpr uint32_t TILE = THREADS * K;
     const uint32_t lane = lane_id();
     const uint8_t *in = buf.in;
-    const uint64_t n = buf.n;
+    const int nb = st.nb;
 
-    // 1. codes and bit count
+    // 1. codes and bit count; entries past the buffer's end are 0 (length 0: they append
+    // nothing)
     const uint64_t tile_start = (uint64_t)tile * TILE;
-    const uint64_t mine = tile_start + (uint64_t)threadIdx.x * K;
-    const int nb = mine >= n ? 0 : (n - mine >= (uint64_t)K ? K : (int)(n - mine));
     uint32_t e[K];
     uint32_t bits = 0;
     if (nb == K) {
-        const u32x4 *p = reinterpret_cast<const u32x4 *>(in + mine);
 #pragma unroll
-        for (int v = 0; v < K / 16; v++) {
-            const u32x4 q = __builtin_nontemporal_load(&p[v]);
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                e[16 * v + j] = s_tab[(q[j >> 2] >> (8 * (j & 3))) & 0xFFu];
-                bits += e[16 * v + j] & 31u;
-            }
+        for (int j = 0; j < K; j++) {
+            e[j] = s_tab[enc_byte<K>(st, j)];
+            bits += e[j] & 31u;
         }
     } else {
 #pragma unroll
         for (int j = 0; j < K; j++) {
-            e[j] = 0u;  // length 0: appends nothing
-            if (j < nb) e[j] = s_tab[in[mine + j]];
+            e[j] = j < nb ? s_tab[enc_byte<K>(st, j)] : 0u;
             bits += e[j] & 31u;
         }
     }
@@ -298,17 +341,22 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     // 2. tile scan, zero the used part of the LDS images, look-back
     uint32_t tile_bits;
     const uint32_t excl = block_excl_sum<THREADS>(bits, s_wave, tile_bits);
-    const uint32_t used_words = (tile_bits >> 5) + 8, used_gap = (tile_bits >> 10) + 3;
+    const uint32_t used_words = (tile_bits >> 5) + 8, used_gap = (tile_bits >> 9) + 5;
     for (uint32_t i = threadIdx.x; i < used_words; i += THREADS) s_out[i] = 0u;
     for (uint32_t i = threadIdx.x; i < used_gap; i += THREADS) s_gap[i] = 0u;
     if (threadIdx.x < 64) {
         const uint64_t E = look_back(buf.status, tile, tile_bits, wpl);
-        if (threadIdx.x == 0) *s_excl = E;
+        if (threadIdx.x == 0) {
+            *s_excl = E;
+            if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
+        }
     }
     __syncthreads();
     const uint64_t E = *s_excl;
-    const uint64_t base_w = (E >> 5) & ~3ull;  // global word of s_out[0]
-    const uint64_t gap_base = E >> 10;         // global gap word of s_gap[0]
+    const uint64_t base_w = (E >> 5) & ~3ull;  // global word of s_out[0], 4 x global segment E/128
+    const uint64_t gap_base = E >> 10;         // global gap word of bytes 0..7 of s_gap
+    // one spill byte per segment: byte (E/128 mod 8) + i of s_gap is segment E/128 + i
+    uint8_t *s_spill = reinterpret_cast<uint8_t *>(s_gap) + ((E >> 7) & 7u);
 
     // 3. pack, two codes at a time: a pair is at most 32 bits, so appending it to at most 31
     // pending bits completes at most one word
@@ -328,14 +376,13 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
                 // the zeroed image takes an OR for every word: only a thread's first word is
                 // shared with its predecessor, and OR needs no branch on which one it is
                 atomicOr(&s_out[w], (uint32_t)(acc >> nacc));
-                if ((w & 3u) == 3u && nacc != 0) {
-                    // nacc bits lie past the segment boundary: all of them belong to the second
+                if ((w & 3u) == 3u) {
+                    // word w closes a segment (its thread is the one holding the segment's last
+                    // bit); nacc bits lie past the boundary: all of them belong to the second
                     // code if it straddles, else the first code straddles by nacc - l2 (0 when
-                    // the first code ends exactly on the boundary)
-                    const uint32_t spill =
-                        (uint32_t)nacc >= l2 ? (uint32_t)nacc - l2 : (uint32_t)nacc;
-                    const uint64_t seg = (base_w + w) >> 2;
-                    if (spill) atomicOr(&s_gap[(seg >> 3) - gap_base], spill << ((seg & 7) * 4));
+                    // it ends exactly on the boundary)
+                    s_spill[w >> 2] =
+                        (uint8_t)((uint32_t)nacc >= l2 ? (uint32_t)nacc - l2 : (uint32_t)nacc);
                 }
                 w++;
             }
@@ -343,6 +390,12 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         if (nacc > 0) atomicOr(&s_out[w], (uint32_t)(acc << (32 - nacc)));
     }
     __syncthreads();
+    // the next tile's input loads overlap this tile's stores (issued here, where the codes of
+    // this tile no longer occupy registers)
+    if (next.counter) {
+        next.tile = *next.s_ticket;
+        next_stage = enc_fetch<THREADS, K>(buf, next.tile);
+    }
 
     // 4a. first word: add the previous tile's last E%32 bits
     const uint32_t r = (uint32_t)(E & 31);
@@ -381,12 +434,15 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         }
     }
 
-    // 5. gap words
+    // 5. gap words: eight spill bytes -> eight nibbles, low nibble first
     if (T > 0) {
         const uint64_t g_hi = (E + T - 1) >> 10;
         for (uint64_t g = gap_base + threadIdx.x; g <= g_hi; g += THREADS) {
             if (g >= buf.gap_words) break;
-            const uint32_t val = s_gap[g - gap_base];
+            const uint32_t lo = s_gap[2 * (g - gap_base)], hi = s_gap[2 * (g - gap_base) + 1];
+            // bytes b0..b3 (each < 16) -> (b0 | b1 << 4) in byte 0 and (b2 | b3 << 4) in byte 2
+            const uint32_t tl = lo | (lo >> 4), th = hi | (hi >> 4);
+            const uint32_t val = __builtin_amdgcn_perm(th, tl, 0x06040200u);
             const bool inside = (g * 1024 + 128 > E) && ((g + 1) * 1024 <= E + T);
             if (inside)
                 buf.gap[g] = val;
@@ -409,12 +465,18 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
     uint32_t *s_out = s_dyn;
     uint32_t *s_gap = s_dyn + out_words;
     for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = table.e[i];
+    // (the input is loaded at the start of each tile: holding the next tile's bytes as well
+    // would cost more waves per SIMD than the hidden latency gains)
+    NextTile none{nullptr, nullptr, 0u};
     while (true) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= buf.num_tiles) break;
-        enc_tile<THREADS, K>(buf, tile, s_tab, s_out, s_gap, s_wave, &s_excl, wpl);
+        const EncStage<K> st = enc_fetch<THREADS, K>(buf, tile);
+        EncStage<K> unused;
+        enc_tile<THREADS, K>(buf, tile, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl, none,
+                             unused);
     }
 }
 
@@ -464,29 +526,48 @@ __device__ __forceinline__ uint32_t dec_lookup(const uint32_t *s_tab, uint32_t h
     return ent;
 }
 
+// A lane's share of a tile's global inputs: its segment's 4 payload words, one of the words
+// after the tile's last segment (lanes 0..7), and the gap word holding its start offset.  Loaded
+// into registers one tile ahead, so their HBM latency overlaps the previous tile's work.
+struct DecStage {
+    u32x4 q;
+    uint32_t extra;
+    uint32_t gapw;
+};
+
+template <int THREADS>
+__device__ __forceinline__ DecStage dec_fetch(const DecBuffer &buf, uint32_t tile) {
+    DecStage st;
+    st.q = u32x4{0u, 0u, 0u, 0u};
+    st.extra = 0u;
+    st.gapw = 0u;
+    if (tile >= buf.num_tiles) return st;
+    const uint64_t w_base = 4ull * tile * THREADS;
+    const uint64_t wi = w_base + 4ull * threadIdx.x;
+    if (wi + 4 <= buf.payload_words) {
+        st.q = *reinterpret_cast<const u32x4 *>(buf.payload + wi);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) st.q[i] = wi + i < buf.payload_words ? buf.payload[wi + i] : 0u;
+    }
+    if (threadIdx.x < 8) {
+        const uint64_t x = w_base + 4ull * THREADS + threadIdx.x;
+        st.extra = x < buf.payload_words ? buf.payload[x] : 0u;
+    }
+    const uint32_t seg = tile * THREADS + threadIdx.x;
+    if (seg > 0 && seg < buf.gap_elements) st.gapw = buf.gap[(seg - 1) >> 3];
+    return st;
+}
+
 template <int THREADS, int NDW, bool SINGLE>
-__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, DecTable t,
-                                         const uint32_t *s_tab, uint32_t *s_pay,
+__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, const DecStage &st,
+                                         DecTable t, const uint32_t *s_tab, uint32_t *s_pay,
                                          uint8_t *s_bytes, uint32_t *s_wave, uint64_t *s_base,
-                                         uint32_t wpl) {
+                                         uint32_t wpl, NextTile &next, DecStage &next_stage) {
     const uint32_t P = t.prefix_bits;
     // 0. stage the tile's payload words (THREADS segments x 4 words + the words after them)
-    const uint64_t w_base = 4ull * tile * THREADS;
-    {
-        const uint64_t wi = w_base + 4ull * threadIdx.x;
-        u32x4 q = {0u, 0u, 0u, 0u};
-        if (wi + 4 <= buf.payload_words) {
-            q = *reinterpret_cast<const u32x4 *>(buf.payload + wi);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; i++) q[i] = wi + i < buf.payload_words ? buf.payload[wi + i] : 0u;
-        }
-        *reinterpret_cast<u32x4 *>(s_pay + 4 * threadIdx.x) = q;
-        if (threadIdx.x < 8) {
-            const uint64_t x = w_base + 4ull * THREADS + threadIdx.x;
-            s_pay[4 * THREADS + threadIdx.x] = x < buf.payload_words ? buf.payload[x] : 0u;
-        }
-    }
+    *reinterpret_cast<u32x4 *>(s_pay + 4 * threadIdx.x) = st.q;
+    if (threadIdx.x < 8) s_pay[4 * THREADS + threadIdx.x] = st.extra;
     __syncthreads();
 
     // 1. decode this lane's segment into its slot.  Bit positions are relative to the
@@ -498,8 +579,8 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     uint32_t cnt = 0;
     uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
     if (seg < buf.gap_elements) {
-        uint32_t p = 0;
-        if (seg > 0) p = (buf.gap[(seg - 1) >> 3] >> (((seg - 1) & 7u) * 4u)) & 15u;
+        const uint32_t p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
+        uint32_t p = p0;
         const uint64_t avail = 32ull * buf.payload_words - 128ull * seg;
         const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
         const uint32_t *pw = s_pay + 4 * threadIdx.x;
@@ -545,10 +626,17 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, De
     }
     if (threadIdx.x < 64) {
         const uint64_t O = look_back(buf.status, tile, tile_syms, wpl);
-        if (threadIdx.x == 0) *s_base = O;
+        if (threadIdx.x == 0) {
+            *s_base = O;
+            if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
+        }
     }
     __syncthreads();
     const uint64_t O = *s_base;
+    if (next.counter) {
+        next.tile = *next.s_ticket;
+        next_stage = dec_fetch<THREADS>(buf, next.tile);
+    }
 
     // 3. compact into LDS, aligned like the output (byte a of the image = output byte
     // (O & ~15) + a).  A lane's bytes are [a, end); its words strictly inside are written
@@ -631,12 +719,16 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
     uint32_t *s_pay = s_dyn + slot_offset;  // 16-byte aligned; dec_payload_words(THREADS)
     uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_pay + dec_payload_words(THREADS));
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
-    while (true) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= buf.num_tiles) break;
-        dec_tile<THREADS, NDW, SINGLE>(buf, tile, t, s_tab, s_pay, s_bytes, s_wave, &s_base, wpl);
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    NextTile next{counter, &s_tile, s_tile};
+    DecStage st = dec_fetch<THREADS>(buf, next.tile);
+    while (next.tile < buf.num_tiles) {
+        const uint32_t cur = next.tile;
+        DecStage sn;
+        dec_tile<THREADS, NDW, SINGLE>(buf, cur, st, t, s_tab, s_pay, s_bytes, s_wave, &s_base,
+                                       wpl, next, sn);
+        st = sn;
     }
 }
 
@@ -782,7 +874,11 @@ __global__ __launch_bounds__(THREADS) void enc_batch_kernel(
         buf.status = status + job.base;
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
-        enc_tile<THREADS, K>(buf, g - job.base, s_tab, s_out, s_gap, s_wave, &s_excl, wpl);
+        const EncStage<K> st = enc_fetch<THREADS, K>(buf, g - job.base);
+        NextTile none{nullptr, nullptr, 0u};
+        EncStage<K> unused;
+        enc_tile<THREADS, K>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
+                             none, unused);
     }
 }
 
@@ -844,8 +940,11 @@ __global__ __launch_bounds__(THREADS) void dec_batch_kernel(
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
         const DecTable t{job.prefix_bits, job.l1_entries};
-        dec_tile<THREADS, NDW, false>(buf, g - job.base, t, s_tab, s_pay, s_bytes, s_wave,
-                                      &s_base, wpl);
+        const DecStage st = dec_fetch<THREADS>(buf, g - job.base);
+        NextTile none{nullptr, nullptr, 0u};
+        DecStage unused;
+        dec_tile<THREADS, NDW, false>(buf, g - job.base, st, t, s_tab, s_pay, s_bytes, s_wave,
+                                      &s_base, wpl, none, unused);
     }
 }
 
