@@ -103,25 +103,48 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate,
-                              uint32_t wpl) {
+// The look-back in two halves, so that its first round trip overlaps other work of the
+// workgroup: lb_start publishes the tile's aggregate (the inclusive prefix for tile 0) and
+// issues the loads of the first window; lb_finish completes the walk and publishes the
+// inclusive prefix.  Both are called by all 64 lanes of one wave.
+struct LookBack {
+    uint64_t v[kMaxWindowPerLane];
+};
+
+__device__ __forceinline__ void lb_start(uint64_t *status, uint32_t tile, uint64_t aggregate,
+                                         uint32_t wpl, LookBack &lb) {
     const uint32_t lane = lane_id();
     if (tile == 0) {
         if (lane == 0) status_store(&status[0], kFlagP | aggregate);
-        return 0;
+        return;
     }
     if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
+    const int64_t first = (int64_t)tile - 1 - (int64_t)(lane * wpl);
+#pragma unroll
+    for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
+        const int64_t idx = first - (int64_t)j;
+        // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
+        lb.v[j] = j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP);
+    }
+}
+
+__device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregate, uint32_t wpl,
+                              LookBack &lb) {
+    if (tile == 0) return 0;
+    const uint32_t lane = lane_id();
     uint64_t excl = 0;
     int64_t end = (int64_t)tile - 1;
+    bool loaded = true;  // the first window was loaded by lb_start
     while (true) {
         const int64_t first = end - (int64_t)(lane * wpl);
         uint64_t v[kMaxWindowPerLane];
 #pragma unroll
         for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
             const int64_t idx = first - (int64_t)j;
-            // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
-            v[j] = j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP);
+            v[j] = loaded ? lb.v[j]
+                          : (j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP));
         }
+        loaded = false;
         while (true) {
             bool missing = false;
 #pragma unroll
@@ -153,6 +176,13 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregat
     return excl;
 }
 
+__device__ __forceinline__ uint64_t look_back(uint64_t *status, uint32_t tile,
+                                              uint64_t aggregate, uint32_t wpl) {
+    LookBack lb;
+    lb_start(status, tile, aggregate, wpl, lb);
+    return lb_finish(status, tile, aggregate, wpl, lb);
+}
+
 // ---------------------------------------------------------------------------------------------
 // zero fill (used for the histogram, the gap array and the look-back workspace)
 // Where a tile claims the workgroup's next tile: after its own look-back, so tiles are claimed
@@ -162,6 +192,7 @@ struct NextTile {
     uint32_t *counter;  // ticket counter; nullptr = the caller claims tiles itself
     uint32_t *s_ticket; // LDS word broadcasting the ticket
     uint32_t tile;      // out: the claimed ticket
+    uint8_t *s_stage;   // encoder: LDS staging area the next tile's input is copied into
 };
 
 __global__ void fill_zero_kernel(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb) {
@@ -308,12 +339,43 @@ __device__ __forceinline__ uint32_t enc_byte(const EncStage<K> &st, int j) {
     return (st.q[j >> 4][(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
 }
 
+// Whether every input byte of `tile` lies inside the buffer (only such tiles are staged
+// through LDS: the copies move whole 16-byte pieces).
+template <int THREADS, int K>
+__device__ __forceinline__ bool enc_tile_full(const EncBuffer &buf, uint32_t tile) {
+    return tile < buf.num_tiles && (uint64_t)(tile + 1) * (THREADS * K) <= buf.n;
+}
+
+// Starts copying a full tile's input from HBM into LDS without passing through registers
+// (global_load_lds_dwordx4): piece v of thread t lands at s_stage + 16 * (v * THREADS + t), so
+// every lane later reads back exactly the pieces its own load wrote.
+template <int THREADS, int K>
+__device__ __forceinline__ void enc_stage_async(const EncBuffer &buf, uint32_t tile,
+                                                uint8_t *s_stage) {
+    const uint8_t *src = buf.in + (uint64_t)tile * (THREADS * K) + (uint64_t)threadIdx.x * K;
+    const uint32_t wave = threadIdx.x / 64u;
+#pragma unroll
+    for (int v = 0; v < K / 16; v++)
+        __builtin_amdgcn_global_load_lds(src + 16 * v, s_stage + 16u * (v * THREADS + wave * 64u),
+                                         16, 0, 0);
+}
+
+template <int THREADS, int K>
+__device__ __forceinline__ EncStage<K> enc_read_stage(const uint8_t *s_stage) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+    EncStage<K> st;
+#pragma unroll
+    for (int v = 0; v < K / 16; v++)
+        st.q[v] = *reinterpret_cast<const u32x4 *>(s_stage + 16u * (v * THREADS + threadIdx.x));
+    st.nb = K;
+    return st;
+}
+
 template <int THREADS, int K>
 __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
                                          const EncStage<K> &st, const uint32_t *s_tab,
                                          uint32_t *s_out, uint32_t *s_gap, uint32_t *s_wave,
-                                         uint64_t *s_excl, uint32_t wpl, NextTile &next,
-                                         EncStage<K> &next_stage) {
+                                         uint64_t *s_excl, uint32_t wpl, NextTile &next) {
     constexpr uint32_t TILE = THREADS * K;
     const uint32_t lane = lane_id();
     const uint8_t *in = buf.in;
@@ -353,6 +415,11 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     }
     __syncthreads();
     const uint64_t E = *s_excl;
+    if (next.counter) {
+        next.tile = *next.s_ticket;
+        if (next.s_stage && enc_tile_full<THREADS, K>(buf, next.tile))
+            enc_stage_async<THREADS, K>(buf, next.tile, next.s_stage);
+    }
     const uint64_t base_w = (E >> 5) & ~3ull;  // global word of s_out[0], 4 x global segment E/128
     const uint64_t gap_base = E >> 10;         // global gap word of bytes 0..7 of s_gap
     // one spill byte per segment: byte (E/128 mod 8) + i of s_gap is segment E/128 + i
@@ -390,12 +457,6 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         if (nacc > 0) atomicOr(&s_out[w], (uint32_t)(acc << (32 - nacc)));
     }
     __syncthreads();
-    // the next tile's input loads overlap this tile's stores (issued here, where the codes of
-    // this tile no longer occupy registers)
-    if (next.counter) {
-        next.tile = *next.s_ticket;
-        next_stage = enc_fetch<THREADS, K>(buf, next.tile);
-    }
 
     // 4a. first word: add the previous tile's last E%32 bits
     const uint32_t r = (uint32_t)(E & 31);
@@ -453,10 +514,18 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     __syncthreads();
 }
 
-template <int THREADS, int K>
+// LDS words before the input staging area of the plain encoder (16-byte aligned)
+__host__ __device__ constexpr uint32_t enc_stage_word(uint32_t out_words, uint32_t gap_words) {
+    return (out_words + gap_words + 3u) & ~3u;
+}
+
+// STAGE: the next tile's input is copied into LDS (a tile's worth of bytes after the images)
+// while the current tile packs and stores, claimed right after the current tile's look-back.
+template <int THREADS, int K, bool STAGE>
 __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable table,
                                                       uint32_t *__restrict__ counter,
-                                                      uint32_t out_words, uint32_t wpl) {
+                                                      uint32_t out_words, uint32_t stage_word,
+                                                      uint32_t wpl) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_tab[256];
     __shared__ uint32_t s_wave[THREADS / 64];
@@ -464,19 +533,19 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
     __shared__ uint32_t s_tile;
     uint32_t *s_out = s_dyn;
     uint32_t *s_gap = s_dyn + out_words;
+    uint8_t *s_stage = reinterpret_cast<uint8_t *>(s_dyn + stage_word);
     for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = table.e[i];
-    // (the input is loaded at the start of each tile: holding the next tile's bytes as well
-    // would cost more waves per SIMD than the hidden latency gains)
-    NextTile none{nullptr, nullptr, 0u};
-    while (true) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile >= buf.num_tiles) break;
-        const EncStage<K> st = enc_fetch<THREADS, K>(buf, tile);
-        EncStage<K> unused;
-        enc_tile<THREADS, K>(buf, tile, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl, none,
-                             unused);
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    NextTile next{counter, &s_tile, s_tile, STAGE ? s_stage : nullptr};
+    if (STAGE && enc_tile_full<THREADS, K>(buf, next.tile))
+        enc_stage_async<THREADS, K>(buf, next.tile, s_stage);
+    while (next.tile < buf.num_tiles) {
+        const uint32_t cur = next.tile;
+        const EncStage<K> st = (STAGE && enc_tile_full<THREADS, K>(buf, cur))
+                                   ? enc_read_stage<THREADS, K>(s_stage)
+                                   : enc_fetch<THREADS, K>(buf, cur);
+        enc_tile<THREADS, K>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl, next);
     }
 }
 
@@ -611,10 +680,12 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
 #undef HGAP_DEC_STEP
     }
 
-    // 2. scan + look-back
+    // 2. scan; publish the tile's symbol count and start the look-back (its loads complete
+    // while the slots are compacted)
     uint32_t tile_syms;
     const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
-    // pull the slot into registers while wave 0 looks back
+    LookBack lb;
+    if (threadIdx.x < 64) lb_start(buf.status, tile, tile_syms, wpl, lb);
     uint32_t R[NDW + 1];
     const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
 #pragma unroll
@@ -624,24 +695,12 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
         if (__ballot(4u * j < cnt) == 0) break;
         if (4u * j < cnt) R[j] = slot_w[j];
     }
-    if (threadIdx.x < 64) {
-        const uint64_t O = look_back(buf.status, tile, tile_syms, wpl);
-        if (threadIdx.x == 0) {
-            *s_base = O;
-            if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
-        }
-    }
-    __syncthreads();
-    const uint64_t O = *s_base;
-    if (next.counter) {
-        next.tile = *next.s_ticket;
-        next_stage = dec_fetch<THREADS>(buf, next.tile);
-    }
+    __syncthreads();  // every slot is in registers: the slot space is free
 
-    // 3. compact into LDS, aligned like the output (byte a of the image = output byte
-    // (O & ~15) + a).  A lane's bytes are [a, end); its words strictly inside are written
-    // whole, its first and last words (shared with neighbours) are zeroed first and OR-ed.
-    const uint32_t a = (uint32_t)(O & 15u) + excl;
+    // 3. compact into LDS relative to the tile's start: byte 16 + i of the image is the tile's
+    // output byte i.  A lane's bytes are [a, end); its words strictly inside are written whole,
+    // its first and last words (shared with neighbours) are zeroed first and OR-ed.
+    const uint32_t a = 16u + excl;
     const uint32_t end = a + cnt;
     uint32_t *s_words = reinterpret_cast<uint32_t *>(s_bytes);
     uint32_t *dst = s_words + (a >> 2);
@@ -669,23 +728,72 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
                 atomicOr(&dst[j], v & himask);
         }
     }
+    // the tile's output offset, and the workgroup's next tile (claimed after this tile's
+    // look-back, so tiles are claimed in the order their predecessors resolve)
+    if (threadIdx.x < 64) {
+        const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
+        if (threadIdx.x == 0) {
+            *s_base = O;
+            if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
+        }
+    }
     __syncthreads();
+    const uint64_t O = *s_base;
+    if (next.counter) {
+        next.tile = *next.s_ticket;
+        next_stage = dec_fetch<THREADS>(buf, next.tile);
+    }
 
-    // 4. store owned bytes
+    // 4. store owned bytes [O, O + tile_syms) (capped at n) in aligned 16-byte chunks.  Chunk c
+    // starts at output byte (O & ~15) + 16c = image byte 16 + 16c - (O & 15): the 16 bytes at
+    // shift sh = 16 - (O & 15) into image chunks c and c + 1 (sh is the same for every chunk).
     const uint64_t lo = O;
     uint64_t hi = O + tile_syms;
     if (hi > buf.n) hi = buf.n;
     if (hi > lo) {
         const uint64_t base = O & ~15ull;
         const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
+        const uint32_t sh = 16u - (uint32_t)(O & 15u);  // 1..16
+        const uint32_t qw = sh >> 2, rb = sh & 3u;        // whole words (0..4), then bytes
         for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+            const u32x4 x0 = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
+            const u32x4 x1 = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c + 16);
+            const uint32_t x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+            uint32_t w[5];
+            // words qw..qw+4 of the 32-byte window (qw is uniform: a scalar branch)
+            switch (qw) {
+            case 0:
+#pragma unroll
+                for (int k = 0; k < 5; k++) w[k] = x[k];
+                break;
+            case 1:
+#pragma unroll
+                for (int k = 0; k < 5; k++) w[k] = x[1 + k];
+                break;
+            case 2:
+#pragma unroll
+                for (int k = 0; k < 5; k++) w[k] = x[2 + k];
+                break;
+            case 3:
+#pragma unroll
+                for (int k = 0; k < 5; k++) w[k] = x[3 + k];
+                break;
+            default:  // 4: sh = 16, rb = 0
+#pragma unroll
+                for (int k = 0; k < 4; k++) w[k] = x[4 + k];
+                w[4] = 0u;
+                break;
+            }
+            u32x4 v;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                v[k] = rb ? __builtin_amdgcn_alignbyte(w[k + 1], w[k], rb) : w[k];
             const uint64_t g = base + 16ull * c;
             if (g >= lo && g + 16 <= hi) {
-                const u32x4 v = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
                 __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + g));
             } else {
                 for (int i = 0; i < 16; i++)
-                    if (g + i >= lo && g + i < hi) buf.out[g + i] = s_bytes[16 * c + i];
+                    if (g + i >= lo && g + i < hi) buf.out[g + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
             }
         }
     }
@@ -721,7 +829,7 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
-    NextTile next{counter, &s_tile, s_tile};
+    NextTile next{counter, &s_tile, s_tile, nullptr};
     DecStage st = dec_fetch<THREADS>(buf, next.tile);
     while (next.tile < buf.num_tiles) {
         const uint32_t cur = next.tile;
@@ -875,10 +983,9 @@ __global__ __launch_bounds__(THREADS) void enc_batch_kernel(
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
         const EncStage<K> st = enc_fetch<THREADS, K>(buf, g - job.base);
-        NextTile none{nullptr, nullptr, 0u};
-        EncStage<K> unused;
+        NextTile none{nullptr, nullptr, 0u, nullptr};
         enc_tile<THREADS, K>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
-                             none, unused);
+                             none);
     }
 }
 
@@ -941,7 +1048,7 @@ __global__ __launch_bounds__(THREADS) void dec_batch_kernel(
         buf.reserved = 0;
         const DecTable t{job.prefix_bits, job.l1_entries};
         const DecStage st = dec_fetch<THREADS>(buf, g - job.base);
-        NextTile none{nullptr, nullptr, 0u};
+        NextTile none{nullptr, nullptr, 0u, nullptr};
         DecStage unused;
         dec_tile<THREADS, NDW, false>(buf, g - job.base, st, t, s_tab, s_pay, s_bytes, s_wave,
                                       &s_base, wpl, none, unused);
@@ -1070,10 +1177,20 @@ static hgap_status launch_enc_t(const EncBuffer &b, const EncTable &t, uint32_t 
     constexpr uint32_t tile = THREADS * K;
     const uint32_t out_words = enc_out_words(tile, max_len);
     const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
-    auto kern = enc_kernel<THREADS, K>;
+    // with the input staging area when it fits the CU's LDS, else without
+    const uint32_t stage_word = enc_stage_word(out_words, enc_gap_words(tile, max_len));
+    const size_t lds_staged = 4ull * stage_word + tile;
+    auto staged = enc_kernel<THREADS, K, true>;
+    if (allow_dynamic_lds(staged, lds_staged)) {
+        const unsigned grid = grid_for(staged, THREADS, lds_staged, c.blocks_per_cu, b.num_tiles);
+        hipLaunchKernelGGL(staged, dim3(grid), dim3(THREADS), lds_staged, s, b, t, counter,
+                           out_words, stage_word, window_per_lane(c));
+        return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+    }
+    auto kern = enc_kernel<THREADS, K, false>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words, 0u,
                        window_per_lane(c));
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
@@ -1234,7 +1351,8 @@ template <int THREADS, int K>
 static bool enc_shape_fits(uint32_t max_len) {
     constexpr uint32_t tile = THREADS * K;
     const size_t lds = 4ull * (enc_out_words(tile, max_len) + enc_gap_words(tile, max_len));
-    return lds_fits(enc_kernel<THREADS, K>, lds) && lds_fits(enc_batch_kernel<THREADS, K>, lds);
+    return lds_fits(enc_kernel<THREADS, K, false>, lds) &&
+           lds_fits(enc_batch_kernel<THREADS, K>, lds);
 }
 
 static bool enc_fits(const hgap_launch_config &c, uint32_t max_len) {
