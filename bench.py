@@ -69,6 +69,91 @@ def cpu_baseline(d_dev, sample_bytes, repeats, threads):
                 host_cpus=os.cpu_count())
 
 
+class _Hip:
+    """hipMalloc / hipMemcpy through ctypes (the reference-named encode() frees its input with
+    hipFree, so that input cannot be a torch allocation)."""
+
+    def __init__(self):
+        import ctypes
+
+        self.c = ctypes
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int]
+
+    def malloc(self, n):
+        p = self.c.c_void_p()
+        if self.lib.hipMalloc(self.c.byref(p), n) != 0:
+            raise RuntimeError("hipMalloc failed")
+        return p
+
+
+def compat_pcie(d_in, n):
+    """The reference-named entry points, which take host buffers, on the same buffer:
+    encode() (device input -> host payload + gap, encoder.cu:382-457 semantics) and
+    decoder_l1_l2() (host stream -> host bytes, decoder.cu:732-815 semantics).  Their rates
+    include the PCIe copies and the per-call allocations."""
+    import ctypes
+
+    from huffman_coding_gap_arrays_amd import _capi, codec
+    from huffman_coding_gap_arrays_amd._capi import Codetable, Symbol, TableInfo
+
+    lib = _capi.load()
+    hip = _Hip()
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    hist = codec.histogram(d_in).cpu().numpy().astype(np.uint32)
+    syms = (Symbol * 256)()
+    k = lib.store_symbols(vp(hist), syms)
+    srt = (Symbol * 256)()
+    for j, o in enumerate(sorted(range(k), key=lambda j: (syms[j].num, syms[j].symbol))):
+        srt[j] = syms[o]
+    table = (Codetable * 256)()
+    lib.boundary_PM(srt, k, table)
+    bits = lib.get_outputfilesize(srt, k)
+    G, W = (bits + 127) // 128, (bits + 31) // 32
+    host_out = np.zeros(W + (G + 7) // 8, np.uint32)
+    words = (n + 3) // 4
+    times = []
+    for _ in range(2):  # the first call warms up the host buffers
+        d_copy = hip.malloc(4 * (words + 1))
+        hip.lib.hipMemcpy(d_copy, ctypes.c_void_p(d_in.data_ptr()), n, 3)
+        t0 = time.perf_counter()
+        lib.encode(vp(host_out), W, d_copy, n, G, table)  # frees d_copy
+        times.append(time.perf_counter() - t0)
+    t_enc = times[-1]
+    fsyms = (Symbol * 256)()
+    for j in range(k):
+        fsyms[j] = srt[k - 1 - j]
+    info = TableInfo()
+    P = lib.get_table_info(fsyms, k, 10, ctypes.byref(info))
+    size = 4 * info.ptrtable_size + 256 + info.l1table_size + info.l2table_size
+    stage = np.zeros(size, np.uint8)
+    lib.get_twolevel_table(vp(stage), P, fsyms, k, info)
+    inp = np.concatenate([host_out[W:], host_out[:W]])
+    res = np.zeros(4 * words, np.uint8)
+    # the reference decoder prints its own timing line on stdout: keep stdout for the JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            lib.decoder_l1_l2(vp(inp), W, vp(res), n, G, vp(stage), size, P, k, info)
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.dup2(saved, 1)
+        os.close(saved)
+    t_dec = times[-1]
+    ok = bool((res[:n] == d_in.cpu().numpy()).all())
+    return dict(encode_GBps=round(n / t_enc / 1e9, 2), encode_s=round(t_enc, 4),
+                decode_GBps=round(n / t_dec / 1e9, 2), decode_s=round(t_dec, 4),
+                round_trip_exact=ok,
+                note="reference-named entry points with host buffers: PCIe copies and per-call "
+                     "allocations included (uncompressed bytes per second)")
+
+
 def load_traffic(config, kernel):
     """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc (profiles/*.json), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{config}.json")
@@ -89,6 +174,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the CPU baseline (the GPU box's CPU share is 16)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--compat", action="store_true",
+                    help="also time the reference-named host-buffer entry points (PCIe included)")
     ap.add_argument("--batch-count", type=int, default=4096,
                     help="buffers of the batched workload (0 = skip it)")
     args = ap.parse_args()
@@ -196,6 +283,10 @@ def main():
         batched = batch.bench(count=args.batch_count, variants=variants)
         batched["workload"] = (f"{args.batch_count} buffers, sizes log-uniform 4 KiB..64 MiB, "
                                "Zipf s ~ U(0.6, 2.0), own permutation and codebook each")
+    compat = None
+    if args.compat:
+        log("reference-named host-buffer entry points ...")
+        compat = compat_pcie(d_in, n)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -225,6 +316,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "compat_pcie": compat,
         "round_trip_exact": ok,
     }
     print(json.dumps(line), flush=True)
