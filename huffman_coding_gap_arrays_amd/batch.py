@@ -214,7 +214,7 @@ class BatchCodec:
         if len(um):
             slot[live] = np.array([lib.hgap_decode_slot_words(int(m)) for m in um])[
                 np.searchsorted(um, ms[live])]
-        tw = iv["l1_entries"].astype(np.int64) + iv["l2_entries"]
+        tw = iv["bytes"].astype(np.int64) // 4  # words of each table image
         # one LDS slot size per launch: every member's shape then fits when it fits alone
         dkeys = np.stack([d["dec_threads"], d["lookback_window"], slot], 1)
         for key in np.unique(dkeys[live], axis=0):

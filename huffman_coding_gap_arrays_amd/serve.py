@@ -182,10 +182,10 @@ class _Batch:
                 jobs[j]["gap_elements"] = e.gap_elements
                 jobs[j]["prefix_bits"] = info.prefix_bits
                 jobs[j]["l1_entries"] = info.l1_entries
-                jobs[j]["table_words"] = info.l1_entries + info.l2_entries
+                jobs[j]["table_words"] = info.bytes // 4
                 jobs[j]["max_symbols"] = info.max_symbols_per_segment
                 jobs[j]["max_length"] = info.max_length
-                max_tw = max(max_tw, info.l1_entries + info.l2_entries)
+                max_tw = max(max_tw, info.bytes // 4)
                 max_sym = max(max_sym, info.max_symbols_per_segment)
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(members), ctypes.byref(cfg))
             host, d_jobs = _pinned_to_device(jobs, s.device)

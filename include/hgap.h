@@ -200,7 +200,7 @@ typedef struct hgap_dec_job {
     uint32_t gap_elements;
     uint32_t prefix_bits;    /* from the table's hgap_dectable_info */
     uint32_t l1_entries;
-    uint32_t table_words;    /* l1_entries + l2_entries */
+    uint32_t table_words;    /* words of the table image: hgap_dectable_info.bytes / 4 */
     uint32_t max_symbols;    /* max_symbols_per_segment */
     uint32_t max_length;
     uint32_t base;           /* set by hgap_batch_plan_decode */

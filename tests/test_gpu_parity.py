@@ -130,10 +130,9 @@ def test_decode_prefix_splits(codec, orc, prefix):
     table = codec.DeviceTable(enc.codebook, prefix_bits=prefix)
     info = table.info
     assert (info.l2_entries == 0) == (prefix >= info.max_length)
-    ndw = 11 if info.max_symbols_per_segment + 1 <= 44 else (
-        17 if info.max_symbols_per_segment + 1 <= 68 else 33)
+    ndw = codec._lib().hgap_decode_slot_words(info.max_symbols_per_segment)
     # the default entry point falls back to 256-segment tiles before refusing
-    lds = 4 * ((info.l1_entries + info.l2_entries + 3) // 4 * 4) + 4 * (256 * ndw + 12)
+    lds = 4 * ((info.bytes // 4 + 3) // 4 * 4) + 4 * (4 * 256 + 8 + 256 * ndw + 12)
     if lds > 160 * 1024 - 64:
         with pytest.raises(codec.HgapError) as err:
             codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
