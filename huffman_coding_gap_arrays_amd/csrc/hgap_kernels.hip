@@ -218,8 +218,9 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
     __shared__ uint32_t s_bins[kHistCopies * 256];
     for (int i = threadIdx.x; i < kHistCopies * 256; i += kHistThreads) s_bins[i] = 0u;
     __syncthreads();
-    const uint32_t copy = lane_id();
-    uint32_t *mine = s_bins + copy * 256;
+    // bin b of lane l's copy is word 64 b + l: within each 32-lane LDS pass every lane hits
+    // its own bank, whatever bytes the lanes count
+    uint32_t *mine = s_bins + lane_id();
     const uint64_t vecs = n >> 4;
     const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
     const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
@@ -233,24 +234,25 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
         for (int u = 0; u < 4; u++) {
 #pragma unroll
             for (int j = 0; j < 16; j++)
-                atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) ^ copy], 1u);
+                atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
         }
     }
     for (; v < vecs; v += stride) {
         const u32x4 q = __builtin_nontemporal_load(&in4[v]);
 #pragma unroll
         for (int j = 0; j < 16; j++)
-            atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) ^ copy], 1u);
+            atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
     }
     // tail bytes (n % 16) in the first workgroup
     if (blockIdx.x == 0) {
         const uint64_t t = (vecs << 4) + threadIdx.x;
-        if (t < n) atomicAdd(&mine[in[t] ^ copy], 1u);
+        if (t < n) atomicAdd(&mine[(uint32_t)in[t] << 6], 1u);
     }
     __syncthreads();
-    // thread b owns bin b: sum of the 64 copies, then one global atomic per bin per workgroup
+    // thread b owns bin b: sum of the 64 copies (read in a rotated order so that the lanes of a
+    // pass hit different banks), then one global atomic per bin per workgroup
     uint32_t sum = 0;
-    for (int c = 0; c < kHistCopies; c++) sum += s_bins[c * 256 + (threadIdx.x ^ c)];
+    for (int c = 0; c < kHistCopies; c++) sum += s_bins[(threadIdx.x << 6) + ((c + threadIdx.x) & 63u)];
     if (sum) atomicAdd(&hist[threadIdx.x], sum);
 }
 
@@ -877,8 +879,7 @@ __global__ __launch_bounds__(kHistThreads) void hist_batch_kernel(const HistJobD
                                                                   uint32_t count,
                                                                   uint64_t total) {
     __shared__ uint32_t s_bins[kHistCopies * 256];
-    const uint32_t copy = lane_id();
-    uint32_t *mine = s_bins + copy * 256;
+    uint32_t *mine = s_bins + lane_id();  // bin b of this lane's copy: mine[64 b]
     for (int i = threadIdx.x; i < kHistCopies * 256; i += kHistThreads) s_bins[i] = 0u;
     const uint64_t per = (total + gridDim.x - 1) / gridDim.x;
     const uint64_t a = (uint64_t)blockIdx.x * per;
@@ -894,25 +895,26 @@ __global__ __launch_bounds__(kHistThreads) void hist_batch_kernel(const HistJobD
         const uint64_t vlo = (lo + 15) & ~15ull, vhi = hi & ~15ull;
         if (vlo >= vhi) {
             for (uint64_t i = lo + threadIdx.x; i < hi; i += kHistThreads)
-                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+                atomicAdd(&mine[(uint32_t)job.in[i] << 6], 1u);
         } else {
             for (uint64_t i = lo + threadIdx.x; i < vlo; i += kHistThreads)
-                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+                atomicAdd(&mine[(uint32_t)job.in[i] << 6], 1u);
             for (uint64_t i = vhi + threadIdx.x; i < hi; i += kHistThreads)
-                atomicAdd(&mine[job.in[i] ^ copy], 1u);
+                atomicAdd(&mine[(uint32_t)job.in[i] << 6], 1u);
             const u32x4 *in4 = reinterpret_cast<const u32x4 *>(job.in);
             for (uint64_t v = (vlo >> 4) + threadIdx.x; v < (vhi >> 4); v += kHistThreads) {
                 const u32x4 q = __builtin_nontemporal_load(&in4[v]);
 #pragma unroll
                 for (int k = 0; k < 16; k++)
-                    atomicAdd(&mine[((q[k >> 2] >> (8 * (k & 3))) & 0xFFu) ^ copy], 1u);
+                    atomicAdd(&mine[((q[k >> 2] >> (8 * (k & 3))) & 0xFFu) << 6], 1u);
             }
         }
         __syncthreads();
         uint32_t sum = 0;
         for (int c = 0; c < kHistCopies; c++) {
-            sum += s_bins[c * 256 + (threadIdx.x ^ c)];
-            s_bins[c * 256 + (threadIdx.x ^ c)] = 0u;
+            const uint32_t w = (threadIdx.x << 6) + ((c + threadIdx.x) & 63u);
+            sum += s_bins[w];
+            s_bins[w] = 0u;
         }
         if (sum) atomicAdd(&job.hist[threadIdx.x], sum);
         __syncthreads();
