@@ -102,7 +102,7 @@ def compat_pcie(d_in, n):
     lib = _capi.load()
     hip = _Hip()
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    hist = codec.histogram(d_in).cpu().numpy().astype(np.uint32)
+    hist = codec.host_histogram(codec.histogram(d_in))
     syms = (Symbol * 256)()
     k = lib.store_symbols(vp(hist), syms)
     srt = (Symbol * 256)()
@@ -196,7 +196,7 @@ def main():
     codec.synth_fill(d_in, seed, cdf, perm)
 
     # sizes/buffers from one untimed pass
-    hist = codec.histogram(d_in).cpu().numpy()
+    hist = codec.host_histogram(codec.histogram(d_in))
     cb = codec.codebook_build(hist)
     sizes = codec.encode_sizes(cb, hist)
     W, G, GW = int(sizes.payload_words), int(sizes.gap_elements), int(sizes.gap_words)
@@ -217,7 +217,7 @@ def main():
         h = codec.histogram(d_in, stream)
         if record:
             e[1].record(stream)
-        hb = h.cpu().numpy()  # D2H of 256 counts (synchronizes), as the reference does
+        hb = codec.host_histogram(h)  # D2H of 256 counts (synchronizes), as the reference does
         book = codec.codebook_build(hb)
         sz = codec.encode_sizes(book, hb)
         table = codec.DeviceTable(book, device=dev)

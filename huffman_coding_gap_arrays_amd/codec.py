@@ -94,6 +94,12 @@ def histogram(d_in, stream=None):
     return hist
 
 
+def host_histogram(hist):
+    """Counts of a histogram tensor on the host as uint32 (the device tensor is int32: counts of
+    2^31 and more read back as negative numbers unless reinterpreted)."""
+    return hist.detach().cpu().numpy().view(np.uint32)
+
+
 def codebook_build(hist):
     """Codebook from 256 counts, bit-exact with the reference's boundary_PM."""
     h = _hist_array(hist)

@@ -446,7 +446,7 @@ def run_sweep(spec, reps=5, log=None):
         b = pool[:d["n"]]
         codec.synth_fill(b, d["seed"], synth.cdf_from_pmf(_pmf(d["kind"], d["param"])),
                          synth.permutation(d["perm_seed"]))
-        hist = codec.histogram(b).cpu().numpy()
+        hist = codec.host_histogram(codec.histogram(b))
         cb = codec.codebook_build(hist)
         sz = codec.encode_sizes(cb, hist)
         X[i] = codebook_features(hist, cb)

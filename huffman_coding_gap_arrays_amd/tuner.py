@@ -130,7 +130,7 @@ def main():
     n, seed, cdf, perm = synth.spec(args.config)
     d_in = torch.empty(n, dtype=torch.uint8, device="cuda")
     codec.synth_fill(d_in, seed, cdf, perm)
-    hist = codec.histogram(d_in).cpu().numpy()
+    hist = codec.host_histogram(codec.histogram(d_in))
     cb = codec.codebook_build(hist)
     sizes = codec.encode_sizes(cb, hist)
     C = 4 * (sizes.payload_words + sizes.gap_words)

@@ -191,7 +191,7 @@ def test_full_size_configs(codec, orc, golden_codebooks, name):
     d_dev = device_data(codec, n, seed, cdf, perm)
     g = golden_codebooks
     i = list(g["tag"]).index(name)
-    h = codec.histogram(d_dev).cpu().numpy().astype(np.uint64)
+    h = codec.host_histogram(codec.histogram(d_dev)).astype(np.uint64)
     assert (h == g["hist"][i]).all()
     enc = codec.encode(d_dev, hist=h)
     k = int(g["k"][i])
@@ -313,3 +313,26 @@ def test_cli_tools_round_trip(cuda, orc, tmp_path):
     r = subprocess.run([os.path.join(bindir, "hgap_decoder"), str(tmp_path / "missing")],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+@pytest.mark.slow
+def test_largest_stream_size(codec):
+    """n = 2^32 - 1, the largest size the stream header's u32 field holds: histogram counts sum
+    to n, sizes are consistent, and the decode returns the input byte for byte (a size-free
+    property; the oracle would take minutes at this size)."""
+    import torch
+
+    n = (1 << 32) - 1
+    say("largest stream size", n)
+    cdf, perm = dist_spec("skew")
+    d_dev = device_data(codec, n, 17, cdf, perm)
+    h = codec.host_histogram(codec.histogram(d_dev)).astype(np.uint64)
+    assert int(h.sum()) == n and int(h.max()) >= 1 << 31  # byte 0: ~0.9 n, past int32
+    enc = codec.encode(d_dev, hist=h)
+    sz = enc.sizes
+    assert sz.payload_words == (int(sz.bits) + 31) // 32 and sz.gap_elements == (int(sz.bits) + 127) // 128
+    table = codec.DeviceTable(enc.codebook)
+    out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
+    assert torch.equal(out[:n], d_dev)
+    del out, enc, d_dev
+    torch.cuda.empty_cache()
