@@ -219,7 +219,7 @@ PROTOTYPES = {
     "hgap_batch_prepare": (_INT, [_P, _U32, _P, _P, _P, _U32]),
     "hgap_dectable_build_batch": (_INT, [_P, _U32, _P, _P, _P, _P, _SZ, _U32]),
     "hgap_config_supported": (_INT, [ctypes.c_void_p, _U32, ctypes.POINTER(DectableInfo)]),
-    "hgap_decode_slot_words": (_U32, [_U32]),
+    "hgap_decode_image_bytes": (_U32, [_U32, _U32, _U32]),
     "hgap_mlp_param_count": (_U32, [_U32, _U32, _U32]),
     "hgap_mlp_forward": (_INT, [_P, _U32, _U32, _U32, _U32, _P, _P, _P, _P]),
     "hgap_mlp_backward": (

@@ -209,14 +209,10 @@ class BatchCodec:
                                           ws=self._workspace(total), members=idx))
         self.dec_launches = []
         ms = iv["max_symbols_per_segment"].astype(np.int64)
-        um = np.unique(ms[live]) if len(live) else np.zeros(0, np.int64)
-        slot = np.zeros(len(ms), np.int64)
-        if len(um):
-            slot[live] = np.array([lib.hgap_decode_slot_words(int(m)) for m in um])[
-                np.searchsorted(um, ms[live])]
         tw = iv["bytes"].astype(np.int64) // 4  # words of each table image
-        # one LDS slot size per launch: every member's shape then fits when it fits alone
-        dkeys = np.stack([d["dec_threads"], d["lookback_window"], slot], 1)
+        # one launch per decoder shape; its LDS image is sized from the members' largest table
+        # and densest segments (tiles that outgrow it take the kernel's global-store path)
+        dkeys = np.stack([d["dec_threads"], d["lookback_window"]], 1)
         for key in np.unique(dkeys[live], axis=0):
             idx = live[(dkeys[live] == key).all(1)]
             cfg = _cfg_struct(dict(dec_threads=key[0], lookback_window=key[1]))

@@ -126,9 +126,15 @@ void package_merge_lengths(const uint64_t *w, int k, uint8_t *len) {
     pm->run(len);
 }
 
+// Direct decode-table entry: sym1 | bits << 8 | sym2 << 16 | symbols << 24 (one codeword:
+// bits = its length, symbols = 1; two codewords: bits = len1 + len2, symbols = 2).
 static inline uint32_t single_entry(uint32_t sym, uint32_t len) {
-    return sym | (len << 8) | (len << 24);
+    return sym | (len << 8) | (1u << 24);
 }
+static inline uint32_t pair_entry(uint32_t sym1, uint32_t sym2, uint32_t bits) {
+    return sym1 | (bits << 8) | (sym2 << 16) | (2u << 24);
+}
+static inline uint32_t entry_bits(uint32_t e) { return (e >> 8) & 0xFFu; }
 
 static bool assign_codes(hgap_codebook *cb) {
     // package_merge.cpp:168-181 / decoder/src/get_table.cpp:17-20: code_0 = 0,
@@ -313,12 +319,15 @@ hgap_status hgap_encode_sizes(const hgap_codebook *cb, const uint32_t hist[256],
 // Decode table image.  The first level is indexed by the next `prefix_bits` stream bits.  An
 // entry is either a pointer (bit 31 set: width << 24 | offset) to a second-level block indexed
 // by the following `width` bits, or a direct entry
-//     sym1 | total << 8 | two << 13 | sym2 << 16 | len1 << 24
-// where len1 is the first codeword's length and, when `two` is set, a second complete
-// codeword (sym2) also fits inside the indexed bits and `total` = len1 + len2 (else total =
-// len1).  Second-level entries are single codewords.  Unused entries (incomplete codes) decode
-// as the first symbol with the longest length, so decoding always advances by at least
-// min_length bits.
+//     sym1 | bits << 8 | sym2 << 16 | symbols << 24
+// holding one codeword (symbols = 1, bits = its length) or, when a second complete codeword
+// (sym2) also fits inside the indexed bits, two (symbols = 2, bits = len1 + len2).  The
+// decoder adds `bits` to its position and `symbols` to its output count without decoding the
+// fields further.  Second-level entries are single codewords.  Unused entries (incomplete
+// codes) decode as the first symbol with the longest length, so decoding always advances by
+// at least min_length bits.  After the two levels come kDecLengthWords words of code lengths,
+// one byte per symbol value: the decoder reads the length of a pair's second codeword there
+// when a segment's last step has to drop a second codeword that starts past the segment.
 hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
                                 hgap_dectable_info *info, void *image, size_t image_cap) {
     if (!cb || !info) return HGAP_ERR_INVALID_ARG;
@@ -365,7 +374,7 @@ hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
     info->prefix_bits = P;
     info->l1_entries = l1;
     info->l2_entries = l2;
-    info->bytes = 4u * (l1 + l2);
+    info->bytes = 4u * (l1 + l2 + kDecLengthWords);
     info->min_length = minl;
     info->max_length = maxl;
     info->max_symbols_per_segment = (HGAP_SEGMENT_BITS + minl - 1) / minl;
@@ -416,17 +425,19 @@ hgap_status hgap_dectable_build(const hgap_codebook *cb, uint32_t prefix_bits,
         for (uint32_t idx = 0; idx < l1; idx++) {
             const uint32_t e1 = one[idx];
             if (e1 >> 31) continue;
-            const uint32_t len1 = (e1 >> 24) & 31u;
+            const uint32_t len1 = entry_bits(e1);
             if (len1 + minl > P) continue;
             const uint32_t rest = (idx << len1) & (l1 - 1u);
             const uint32_t e2 = one[rest];
             if (e2 >> 31) continue;
-            const uint32_t len2 = (e2 >> 24) & 31u;
+            const uint32_t len2 = entry_bits(e2);
             if (len1 + len2 > P) continue;
-            tab[idx] = (e1 & 0xFFu) | ((len1 + len2) << 8) | (1u << 13) | ((e2 & 0xFFu) << 16) |
-                       (len1 << 24);
+            tab[idx] = pair_entry(e1 & 0xFFu, e2 & 0xFFu, len1 + len2);
         }
     }
+    uint8_t *lengths = reinterpret_cast<uint8_t *>(tab + l1 + l2);
+    std::memset(lengths, 0, 4u * kDecLengthWords);
+    for (uint32_t t = 0; t < k; t++) lengths[tmp.order_symbol[t]] = tmp.order_length[t];
     return HGAP_OK;
 }
 

@@ -23,6 +23,10 @@ constexpr uint32_t kDefaultLookbackWindow = 64;
 constexpr uint32_t kMinEncTileBytes = 256 * 16;
 constexpr uint32_t kMinDecThreads = 256;
 
+// Decode table image: first level, second level, then this many words holding one byte per
+// symbol value: that symbol's code length (0 for bytes absent from the codebook).
+constexpr uint32_t kDecLengthWords = 64;
+
 // Histogram: 256 threads, one private bin array per lane index (64 copies of 256 bins).
 constexpr int kHistThreads = 256;
 

@@ -12,6 +12,7 @@
 // Reference citations are file:line relative to the reference repository root.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 
@@ -105,10 +106,11 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
 
 // The look-back in two halves, so that its first round trip overlaps other work of the
 // workgroup: lb_start publishes the tile's aggregate (the inclusive prefix for tile 0) and
-// issues the loads of the first window; lb_finish completes the walk and publishes the
-// inclusive prefix.  Both are called by all 64 lanes of one wave.
+// issues the load of each lane's first window word (the whole first window when wpl = 1);
+// lb_finish completes the walk and publishes the inclusive prefix.  Both are called by all 64
+// lanes of one wave.
 struct LookBack {
-    uint64_t v[kMaxWindowPerLane];
+    uint64_t v0;
 };
 
 __device__ __forceinline__ void lb_start(uint64_t *status, uint32_t tile, uint64_t aggregate,
@@ -120,12 +122,8 @@ __device__ __forceinline__ void lb_start(uint64_t *status, uint32_t tile, uint64
     }
     if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
     const int64_t first = (int64_t)tile - 1 - (int64_t)(lane * wpl);
-#pragma unroll
-    for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
-        const int64_t idx = first - (int64_t)j;
-        // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
-        lb.v[j] = j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP);
-    }
+    // before tile 0: an empty inclusive prefix
+    lb.v0 = first >= 0 ? status_load(&status[first]) : kFlagP;
 }
 
 __device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregate, uint32_t wpl,
@@ -141,8 +139,10 @@ __device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregat
 #pragma unroll
         for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
             const int64_t idx = first - (int64_t)j;
-            v[j] = loaded ? lb.v[j]
-                          : (j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP));
+            // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
+            v[j] = loaded && j == 0
+                       ? lb.v0
+                       : (j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP));
         }
         loaded = false;
         while (true) {
@@ -556,18 +556,24 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
 //
 // Tile = THREADS consecutive 128-bit segments, one per thread.  Segment s starts at bit
 // 128*s + (gap element s-1) and decodes every codeword that starts inside it (and inside the
-// payload), as in decoder.cu:505-569.  Per tile:
-//   1. each lane decodes its segment once with a 64-bit bit buffer and the two-level table in
-//      LDS, two lookups per refill in one loop, writing symbols into its own LDS slot (slots
-//      are an odd number of words apart so neighbouring lanes hit different banks);
-//      first-level entries may yield two codewords;
-//   2. workgroup scan of the symbol counts, decoupled look-back for the tile's output offset O;
-//   3. lanes pull their slot into registers, then (after a barrier) write it compacted into the
-//      same LDS space at (O mod 16) + prefix, so the LDS image is aligned like the output:
-//      whole words with plain stores, the two words shared with neighbouring lanes zeroed
-//      first and OR-ed (byte selects by v_perm_b32);
-//   4. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
+// payload), as in decoder.cu:505-569.  A lane holds its segment's four payload words and the
+// word after them in registers (loaded one tile ahead).  Per tile:
+//   1. walk 1: each lane counts its segment's codewords (64-bit bit buffer, two-level table in
+//      LDS, up to two codewords per lookup);
+//   2. workgroup scan of the counts; one wave publishes the tile's symbol count and issues the
+//      first look-back loads;
+//   3. walk 2: each lane decodes its segment again and writes its symbols straight into the
+//      tile's output image in LDS (byte 16 + i of the image = output byte i of the tile), while
+//      the look-back loads are in flight.  A step stores two bytes whatever the entry holds, so
+//      a lane leaves one stray byte just past its range, on its successor's first byte: after
+//      a barrier every lane writes its first symbol again;
+//   4. the look-back wave resolves the tile's output offset O and claims the next tile;
+//   5. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
 //      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
+// The image region holds `image_bytes` bytes (chosen by the launcher for occupancy).  A tile
+// whose symbols do not fit skips the image: after its look-back, walk 2 stores its bytes to
+// global memory directly (slow, but only for tiles averaging far more symbols per segment than
+// the region was sized for).
 struct DecBuffer {
     const uint32_t *payload;
     uint32_t payload_words;
@@ -597,9 +603,10 @@ __device__ __forceinline__ uint32_t dec_lookup(const uint32_t *s_tab, uint32_t h
     return ent;
 }
 
-// A lane's share of a tile's global inputs: its segment's 4 payload words, one of the words
-// after the tile's last segment (lanes 0..7), and the gap word holding its start offset.  Loaded
-// into registers one tile ahead, so their HBM latency overlaps the previous tile's work.
+// A lane's share of a tile's global inputs: its segment's 4 payload words, the word after them
+// (lane 63 of each wave only; the other lanes take it from their successor lane), and the gap
+// word holding its start offset.  Loaded into registers one tile ahead, so their HBM latency
+// overlaps the previous tile's work.
 struct DecStage {
     u32x4 q;
     uint32_t extra;
@@ -613,212 +620,266 @@ __device__ __forceinline__ DecStage dec_fetch(const DecBuffer &buf, uint32_t til
     st.extra = 0u;
     st.gapw = 0u;
     if (tile >= buf.num_tiles) return st;
-    const uint64_t w_base = 4ull * tile * THREADS;
-    const uint64_t wi = w_base + 4ull * threadIdx.x;
+    const uint64_t wi = 4ull * tile * THREADS + 4ull * threadIdx.x;
     if (wi + 4 <= buf.payload_words) {
         st.q = *reinterpret_cast<const u32x4 *>(buf.payload + wi);
     } else {
 #pragma unroll
         for (int i = 0; i < 4; i++) st.q[i] = wi + i < buf.payload_words ? buf.payload[wi + i] : 0u;
     }
-    if (threadIdx.x < 8) {
-        const uint64_t x = w_base + 4ull * THREADS + threadIdx.x;
-        st.extra = x < buf.payload_words ? buf.payload[x] : 0u;
-    }
+    if (lane_id() == 63 && wi + 4 < buf.payload_words) st.extra = buf.payload[wi + 4];
     const uint32_t seg = tile * THREADS + threadIdx.x;
     if (seg > 0 && seg < buf.gap_elements) st.gapw = buf.gap[(seg - 1) >> 3];
     return st;
 }
 
-template <int THREADS, int NDW, bool SINGLE>
-__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, const DecStage &st,
-                                         DecTable t, const uint32_t *s_tab, uint32_t *s_pay,
-                                         uint8_t *s_bytes, uint32_t *s_wave, uint64_t *s_base,
-                                         uint32_t wpl, NextTile &next, DecStage &next_stage) {
-    const uint32_t P = t.prefix_bits;
-    // 0. stage the tile's payload words (THREADS segments x 4 words + the words after them)
-    *reinterpret_cast<u32x4 *>(s_pay + 4 * threadIdx.x) = st.q;
-    if (threadIdx.x < 8) s_pay[4 * THREADS + threadIdx.x] = st.extra;
-    __syncthreads();
+// One lane's segment: payload words w0..w4 (w4 = first word of the next segment), bit limit and
+// start offset.  Bit positions are relative to w0; `bb` holds bits [p, fill) MSB-aligned, `nx`
+// the 32 bits after `fill`, then w3/w4 queue behind it.  Every step starts a codeword at
+// p < limit and takes the entry's bits and symbols as they are.  A two-codeword entry whose
+// second codeword starts at or past `limit` belongs to the next segment; p then reaches
+// `limit`, so only a segment's last step can be such an entry.  A step consumes at most 16
+// bits, so after a refill (>= 32 bits buffered) two steps can run before the next.
+struct DecSeg {
+    uint32_t w0, w1, w2, w3, w4;
+    uint32_t p0, limit;
+};
 
-    // 1. decode this lane's segment into its slot.  Bit positions are relative to the
-    // segment's first payload word; `bb` holds bits [p, fill) MSB-aligned, `nx` the 32 bits
-    // after `fill`.  Every step starts a codeword at p < limit; a two-codeword entry keeps its
-    // second codeword only when that one also starts before `limit`.  A step consumes at most
-    // 16 bits, so after a refill (>= 32 bits buffered) two steps can run before the next.
-    const uint32_t seg = tile * THREADS + threadIdx.x;
-    uint32_t cnt = 0;
-    uint8_t *slot = s_bytes + threadIdx.x * (NDW * 4);
-    if (seg < buf.gap_elements) {
-        const uint32_t p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
-        uint32_t p = p0;
-        const uint64_t avail = 32ull * buf.payload_words - 128ull * seg;
-        const uint32_t limit = avail < 128 ? (uint32_t)avail : 128u;
-        const uint32_t *pw = s_pay + 4 * threadIdx.x;
-        uint64_t bb = ((((uint64_t)pw[0]) << 32) | pw[1]) << p;
-        uint32_t fill = 64;
-        uint32_t nx = pw[2];
-#define HGAP_DEC_STEP()                                                                       \
+// walk 1: the number of codewords starting in [p0, limit)
+template <bool SINGLE>
+__device__ __forceinline__ uint32_t dec_count(const DecSeg &g, const uint32_t *s_tab,
+                                              const uint8_t *s_len, uint32_t P, uint32_t l1) {
+    uint32_t p = g.p0, fill = 64, nx = g.w2, nq = g.w3, nq2 = g.w4;
+    uint64_t bb = ((((uint64_t)g.w0) << 32) | g.w1) << p;
+    uint32_t cnt = 0, last = 0;
+#define HGAP_COUNT_STEP()                                                                     \
     {                                                                                         \
-        const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, t.l1_entries);  \
-        const uint32_t len1 = (ent >> 24) & 31u;                                              \
-        const bool two = ((ent >> 13) & 1u) && p + len1 < limit;                              \
-        slot[cnt] = (uint8_t)ent;                                                             \
-        slot[cnt + 1] = (uint8_t)(ent >> 16);                                                 \
-        cnt += two ? 2u : 1u;                                                                 \
-        const uint32_t adv = two ? ((ent >> 8) & 31u) : len1;                                 \
+        const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);            \
+        cnt += ent >> 24;                                                                     \
+        const uint32_t adv = (ent >> 8) & 0xFFu;                                              \
+        p += adv;                                                                             \
+        bb <<= adv;                                                                           \
+        last = ent;                                                                           \
+    }
+    while (p < g.limit) {
+        HGAP_COUNT_STEP()
+        if (p < g.limit) HGAP_COUNT_STEP()
+        if (fill - p < 32u) {
+            bb |= ((uint64_t)nx) << (p + 32u - fill);
+            fill += 32u;
+            nx = nq;
+            nq = nq2;
+        }
+    }
+#undef HGAP_COUNT_STEP
+    // undo the last step's second codeword when it starts at p - len2 >= limit
+    if ((last >> 24) == 2u && p - s_len[(last >> 16) & 0xFFu] >= g.limit) cnt--;
+    return cnt;
+}
+
+// walk 2: the same steps, storing two bytes per step at wp (LDS image or global output).  The
+// bytes a step stores past the lane's count are overwritten by its next step, except after
+// the last step: one stray byte just past the lane's range.
+template <bool SINGLE, typename Ptr>
+__device__ __forceinline__ void dec_write(const DecSeg &g, const uint32_t *s_tab, uint32_t P,
+                                          uint32_t l1, Ptr wp) {
+    uint32_t p = g.p0, fill = 64, nx = g.w2, nq = g.w3, nq2 = g.w4;
+    uint64_t bb = ((((uint64_t)g.w0) << 32) | g.w1) << p;
+#define HGAP_WRITE_STEP()                                                                     \
+    {                                                                                         \
+        const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);            \
+        wp[0] = (uint8_t)ent;                                                                 \
+        wp[1] = (uint8_t)(ent >> 16);                                                         \
+        wp += ent >> 24;                                                                      \
+        const uint32_t adv = (ent >> 8) & 0xFFu;                                              \
         p += adv;                                                                             \
         bb <<= adv;                                                                           \
     }
-        while (p < limit) {
-            HGAP_DEC_STEP()
-            if (p < limit) HGAP_DEC_STEP()
-            if (fill - p < 32u) {
-                bb |= ((uint64_t)nx) << (p + 32u - fill);
-                fill += 32u;
-                nx = pw[fill >> 5];
-            }
+    while (p < g.limit) {
+        HGAP_WRITE_STEP()
+        if (p < g.limit) HGAP_WRITE_STEP()
+        if (fill - p < 32u) {
+            bb |= ((uint64_t)nx) << (p + 32u - fill);
+            fill += 32u;
+            nx = nq;
+            nq = nq2;
         }
-#undef HGAP_DEC_STEP
+    }
+#undef HGAP_WRITE_STEP
+}
+
+// Global-memory writer for walk 2 of a tile that skips the LDS image: stores symbol i of the
+// lane at out[o + i] when i < cnt and o + i < n (the stray byte is never stored).
+struct DecGlobalOut {
+    uint8_t *out;
+    uint64_t o, n;
+    uint32_t i, cnt;
+    __device__ __forceinline__ void put(uint32_t k, uint8_t v) {
+        if (i + k < cnt && o + i + k < n) out[o + i + k] = v;
+    }
+    struct Ref {
+        DecGlobalOut *w;
+        uint32_t k;
+        __device__ __forceinline__ void operator=(uint8_t v) { w->put(k, v); }
+    };
+    __device__ __forceinline__ Ref operator[](uint32_t k) { return Ref{this, k}; }
+    __device__ __forceinline__ DecGlobalOut &operator+=(uint32_t d) {
+        i += d;
+        return *this;
+    }
+};
+
+template <int THREADS, bool SINGLE>
+__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, const DecStage &st,
+                                         DecTable t, const uint32_t *s_tab,
+                                         const uint8_t *s_len, uint8_t *s_img,
+                                         uint32_t image_bytes, uint32_t *s_wave,
+                                         uint64_t *s_base, uint32_t wpl, NextTile &next,
+                                         DecStage &next_stage) {
+    const uint32_t P = t.prefix_bits;
+    const uint32_t seg = tile * THREADS + threadIdx.x;
+    const bool live = seg < buf.gap_elements;
+    DecSeg g;
+    g.w0 = st.q[0];
+    g.w1 = st.q[1];
+    g.w2 = st.q[2];
+    g.w3 = st.q[3];
+    // the word after this segment: the successor lane's first word (lane 63: loaded)
+    g.w4 = __builtin_amdgcn_ds_bpermute(4 * ((lane_id() + 1) & 63u), (int)st.q[0]);
+    if (lane_id() == 63) g.w4 = st.extra;
+    g.p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
+    {
+        const uint64_t avail = live ? 32ull * buf.payload_words - 128ull * seg : 0ull;
+        g.limit = avail < 128 ? (uint32_t)avail : 128u;
     }
 
-    // 2. scan; publish the tile's symbol count and start the look-back (its loads complete
-    // while the slots are compacted)
+    // 1. count; 2. scan, publish the tile's count and start the look-back
+    const uint32_t cnt = live ? dec_count<SINGLE>(g, s_tab, s_len, P, t.l1_entries) : 0u;
     uint32_t tile_syms;
     const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
     LookBack lb;
     if (threadIdx.x < 64) lb_start(buf.status, tile, tile_syms, wpl, lb);
-    uint32_t R[NDW + 1];
-    const uint32_t *slot_w = reinterpret_cast<const uint32_t *>(slot);
-#pragma unroll
-    for (int j = 0; j <= NDW; j++) R[j] = 0u;
-#pragma unroll
-    for (int j = 0; j < NDW; j++) {
-        if (__ballot(4u * j < cnt) == 0) break;
-        if (4u * j < cnt) R[j] = slot_w[j];
-    }
-    __syncthreads();  // every slot is in registers: the slot space is free
 
-    // 3. compact into LDS relative to the tile's start: byte 16 + i of the image is the tile's
-    // output byte i.  A lane's bytes are [a, end); its words strictly inside are written whole,
-    // its first and last words (shared with neighbours) are zeroed first and OR-ed.
-    const uint32_t a = 16u + excl;
-    const uint32_t end = a + cnt;
-    uint32_t *s_words = reinterpret_cast<uint32_t *>(s_bytes);
-    uint32_t *dst = s_words + (a >> 2);
-    const uint32_t jl = cnt ? ((end - 1) >> 2) - (a >> 2) : 0u;  // index of the last word
-    if (cnt) {
-        dst[0] = 0u;
-        dst[jl] = 0u;
-    }
-    __syncthreads();
-    if (cnt) {
-        const uint32_t m = a & 3u;
-        // output word j = lane bytes [4j - m, 4j - m + 4): bytes (4 - m)..(7 - m) of {R[j]:R[j-1]}
-        const uint32_t sel = 0x07060504u - 0x01010101u * m;
-        const uint32_t himask = 0xFFFFFFFFu >> (8u * (3u - ((end - 1) & 3u)));
-        const uint32_t lomask = 0xFFFFFFFFu << (8u * m);
-        const uint32_t v0 = __builtin_amdgcn_perm(R[0], 0u, sel);
-        atomicOr(&dst[0], v0 & lomask & (jl == 0 ? himask : 0xFFFFFFFFu));
-#pragma unroll
-        for (int j = 1; j <= NDW; j++) {
-            if (__ballot((uint32_t)j <= jl) == 0) break;
-            const uint32_t v = __builtin_amdgcn_perm(R[j], R[j - 1], sel);
-            if ((uint32_t)j < jl)
-                dst[j] = v;
-            else if ((uint32_t)j == jl)
-                atomicOr(&dst[j], v & himask);
+    if (16u + tile_syms + 32u <= image_bytes) {  // uniform
+        // 3. decode into the image while the look-back loads are in flight
+        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, s_img + 16 + excl);
+        __syncthreads();
+        if (cnt) {
+            const uint32_t first =
+                dec_lookup<SINGLE>(s_tab, (uint32_t)(((((uint64_t)g.w0) << 32) | g.w1) << g.p0 >> 32),
+                                   P, t.l1_entries);
+            s_img[16 + excl] = (uint8_t)first;
         }
-    }
-    // the tile's output offset, and the workgroup's next tile (claimed after this tile's
-    // look-back, so tiles are claimed in the order their predecessors resolve)
-    if (threadIdx.x < 64) {
-        const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
-        if (threadIdx.x == 0) {
-            *s_base = O;
-            if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
-        }
-    }
-    __syncthreads();
-    const uint64_t O = *s_base;
-    if (next.counter) {
-        next.tile = *next.s_ticket;
-        next_stage = dec_fetch<THREADS>(buf, next.tile);
-    }
-
-    // 4. store owned bytes [O, O + tile_syms) (capped at n) in aligned 16-byte chunks.  Chunk c
-    // starts at output byte (O & ~15) + 16c = image byte 16 + 16c - (O & 15): the 16 bytes at
-    // shift sh = 16 - (O & 15) into image chunks c and c + 1 (sh is the same for every chunk).
-    const uint64_t lo = O;
-    uint64_t hi = O + tile_syms;
-    if (hi > buf.n) hi = buf.n;
-    if (hi > lo) {
-        const uint64_t base = O & ~15ull;
-        const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
-        const uint32_t sh = 16u - (uint32_t)(O & 15u);  // 1..16
-        const uint32_t qw = sh >> 2, rb = sh & 3u;        // whole words (0..4), then bytes
-        for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
-            const u32x4 x0 = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c);
-            const u32x4 x1 = *reinterpret_cast<const u32x4 *>(s_bytes + 16 * c + 16);
-            const uint32_t x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-            uint32_t w[5];
-            // words qw..qw+4 of the 32-byte window (qw is uniform: a scalar branch)
-            switch (qw) {
-            case 0:
-#pragma unroll
-                for (int k = 0; k < 5; k++) w[k] = x[k];
-                break;
-            case 1:
-#pragma unroll
-                for (int k = 0; k < 5; k++) w[k] = x[1 + k];
-                break;
-            case 2:
-#pragma unroll
-                for (int k = 0; k < 5; k++) w[k] = x[2 + k];
-                break;
-            case 3:
-#pragma unroll
-                for (int k = 0; k < 5; k++) w[k] = x[3 + k];
-                break;
-            default:  // 4: sh = 16, rb = 0
-#pragma unroll
-                for (int k = 0; k < 4; k++) w[k] = x[4 + k];
-                w[4] = 0u;
-                break;
-            }
-            u32x4 v;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                v[k] = rb ? __builtin_amdgcn_alignbyte(w[k + 1], w[k], rb) : w[k];
-            const uint64_t g = base + 16ull * c;
-            if (g >= lo && g + 16 <= hi) {
-                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + g));
-            } else {
-                for (int i = 0; i < 16; i++)
-                    if (g + i >= lo && g + i < hi) buf.out[g + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+        // 4. the tile's output offset, and the workgroup's next tile (claimed after this
+        // tile's look-back, so tiles are claimed in the order their predecessors resolve)
+        if (threadIdx.x < 64) {
+            const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
+            if (threadIdx.x == 0) {
+                *s_base = O;
+                if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
             }
         }
+        __syncthreads();
+        const uint64_t O = *s_base;
+        if (next.counter) {
+            next.tile = *next.s_ticket;
+            next_stage = dec_fetch<THREADS>(buf, next.tile);
+        }
+
+        // 5. store owned bytes [O, O + tile_syms) (capped at n) in aligned 16-byte chunks.
+        // Chunk c starts at output byte (O & ~15) + 16c = image byte 16 + 16c - (O & 15): the
+        // 16 bytes at shift sh = 16 - (O & 15) into image chunks c and c + 1 (sh is the same
+        // for every chunk).
+        const uint64_t lo = O;
+        uint64_t hi = O + tile_syms;
+        if (hi > buf.n) hi = buf.n;
+        if (hi > lo) {
+            const uint64_t base = O & ~15ull;
+            const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
+            const uint32_t sh = 16u - (uint32_t)(O & 15u);  // 1..16
+            const uint32_t qw = sh >> 2, rb = sh & 3u;        // whole words (0..4), then bytes
+            for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+                const u32x4 x0 = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
+                const u32x4 x1 = *reinterpret_cast<const u32x4 *>(s_img + 16 * c + 16);
+                const uint32_t x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+                uint32_t w[5];
+                // words qw..qw+4 of the 32-byte window (qw is uniform: a scalar branch)
+                switch (qw) {
+                case 0:
+#pragma unroll
+                    for (int k = 0; k < 5; k++) w[k] = x[k];
+                    break;
+                case 1:
+#pragma unroll
+                    for (int k = 0; k < 5; k++) w[k] = x[1 + k];
+                    break;
+                case 2:
+#pragma unroll
+                    for (int k = 0; k < 5; k++) w[k] = x[2 + k];
+                    break;
+                case 3:
+#pragma unroll
+                    for (int k = 0; k < 5; k++) w[k] = x[3 + k];
+                    break;
+                default:  // 4: sh = 16, rb = 0
+#pragma unroll
+                    for (int k = 0; k < 4; k++) w[k] = x[4 + k];
+                    w[4] = 0u;
+                    break;
+                }
+                u32x4 v;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    v[k] = rb ? __builtin_amdgcn_alignbyte(w[k + 1], w[k], rb) : w[k];
+                const uint64_t gpos = base + 16ull * c;
+                if (gpos >= lo && gpos + 16 <= hi) {
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
+                } else {
+                    for (int i = 0; i < 16; i++)
+                        if (gpos + i >= lo && gpos + i < hi)
+                            buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+                }
+            }
+        }
+    } else {
+        // the tile's symbols exceed the image region: resolve the offset first, then store
+        // each lane's bytes to global memory
+        if (threadIdx.x < 64) {
+            const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
+            if (threadIdx.x == 0) {
+                *s_base = O;
+                if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
+            }
+        }
+        __syncthreads();
+        const uint64_t O = *s_base;
+        if (next.counter) {
+            next.tile = *next.s_ticket;
+            next_stage = dec_fetch<THREADS>(buf, next.tile);
+        }
+        if (cnt) {
+            DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
+            dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
+        }
     }
-    __syncthreads();
+    // The next tile's first LDS write (its walk 2) follows its scan barrier, so this tile's
+    // image reads are complete by then; no barrier is needed here.
 }
 
-// LDS words of the tile's staged payload: 4 per segment, plus the words a lane's bit buffer may
-// prefetch past its last segment (16-byte multiple)
-__host__ __device__ constexpr uint32_t dec_payload_words(uint32_t threads) {
-    return 4u * threads + 8u;
+// LDS bytes of the decoder's image region, from the device's LDS budget: the region that lets
+// 32 / (THREADS / 64) workgroups share a CU next to the table (or fewer when the table is
+// large), capped at what the largest possible tile needs (16 + THREADS * max_symbols + 32,
+// rounded to 16) and at least 16 * THREADS + 48 bytes; 0 when even that does not fit.
+__host__ __device__ constexpr uint32_t dec_image_need(uint32_t threads, uint32_t max_symbols) {
+    return (16u + threads * max_symbols + 32u + 15u) & ~15u;
 }
 
-// LDS bytes of the decode tile staging region: the staged payload, then THREADS slots of NDW
-// words plus 48 bytes (the last 16-byte copy-out chunk may read up to 29 bytes past THREADS x
-// NDW words; bytes past the tile's output are never stored)
-__host__ __device__ constexpr uint32_t dec_stage_bytes(uint32_t threads, uint32_t ndw) {
-    return 4u * (dec_payload_words(threads) + threads * ndw + 12u);
-}
-
-template <int THREADS, int NDW, bool SINGLE>
-__global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
+template <int THREADS, bool SINGLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void dec_kernel(DecBuffer buf, DecTable t,
                                                       const uint32_t *__restrict__ table,
-                                                      uint32_t table_words, uint32_t slot_offset,
+                                                      uint32_t table_words, uint32_t image_offset,
+                                                      uint32_t image_bytes,
                                                       uint32_t *__restrict__ counter,
                                                       uint32_t wpl) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
@@ -826,8 +887,8 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_tile;
     uint32_t *s_tab = s_dyn;
-    uint32_t *s_pay = s_dyn + slot_offset;  // 16-byte aligned; dec_payload_words(THREADS)
-    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_pay + dec_payload_words(THREADS));
+    const uint8_t *s_len = reinterpret_cast<const uint8_t *>(s_tab + table_words - kDecLengthWords);
+    uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);  // 16-byte aligned
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
@@ -836,8 +897,8 @@ __global__ __launch_bounds__(THREADS) void dec_kernel(DecBuffer buf, DecTable t,
     while (next.tile < buf.num_tiles) {
         const uint32_t cur = next.tile;
         DecStage sn;
-        dec_tile<THREADS, NDW, SINGLE>(buf, cur, st, t, s_tab, s_pay, s_bytes, s_wave, &s_base,
-                                       wpl, next, sn);
+        dec_tile<THREADS, SINGLE>(buf, cur, st, t, s_tab, s_len, s_img, image_bytes, s_wave,
+                                  &s_base, wpl, next, sn);
         st = sn;
     }
 }
@@ -1009,18 +1070,18 @@ struct DecJobDev {
     uint32_t reserved;
 };
 
-template <int THREADS, int NDW>
-__global__ __launch_bounds__(THREADS) void dec_batch_kernel(
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void dec_batch_kernel(
     const DecJobDev *__restrict__ jobs, uint32_t count, uint32_t total_tiles,
-    uint32_t slot_offset, uint32_t *__restrict__ counter, uint64_t *__restrict__ status,
-    uint32_t wpl) {
+    uint32_t image_offset, uint32_t image_bytes, uint32_t *__restrict__ counter,
+    uint64_t *__restrict__ status, uint32_t wpl) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_wave[THREADS / 64];
     __shared__ uint64_t s_base;
     __shared__ uint32_t s_tile, s_job;
     uint32_t *s_tab = s_dyn;
-    uint32_t *s_pay = s_dyn + slot_offset;
-    uint8_t *s_bytes = reinterpret_cast<uint8_t *>(s_pay + dec_payload_words(THREADS));
+    uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);
     uint32_t loaded = 0xFFFFFFFFu;
     while (true) {
         if (threadIdx.x == 0) {
@@ -1052,8 +1113,10 @@ __global__ __launch_bounds__(THREADS) void dec_batch_kernel(
         const DecStage st = dec_fetch<THREADS>(buf, g - job.base);
         NextTile none{nullptr, nullptr, 0u, nullptr};
         DecStage unused;
-        dec_tile<THREADS, NDW, false>(buf, g - job.base, st, t, s_tab, s_pay, s_bytes, s_wave,
-                                      &s_base, wpl, none, unused);
+        const uint8_t *s_len =
+            reinterpret_cast<const uint8_t *>(s_tab + job.table_words - kDecLengthWords);
+        dec_tile<THREADS, false>(buf, g - job.base, st, t, s_tab, s_len, s_img, image_bytes,
+                                 s_wave, &s_base, wpl, none, unused);
     }
 }
 
@@ -1209,27 +1272,70 @@ static hgap_status launch_enc(const EncBuffer &b, const EncTable &t, uint32_t *c
     return HGAP_ERR_UNSUPPORTED;
 }
 
-// slot words per lane: maxsym symbols plus one byte written past the last (second-symbol
-// store), rounded to an odd word count so that lanes' slots start in different LDS banks
-static uint32_t dec_slot_words(uint32_t max_symbols) {
-    if (max_symbols + 1 <= 4 * 11) return 11;
-    if (max_symbols + 1 <= 4 * 17) return 17;
-    return 33;
+// Decoder workgroup LDS: the table image (rounded up to 4 words), then the output image.
+static uint32_t dec_image_offset(uint32_t table_words) { return (table_words + 3u) & ~3u; }
+
+template <typename K>
+static size_t static_lds(K kernel) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(kernel)) != hipSuccess) {
+        (void)hipGetLastError();
+        return kMaxLds;
+    }
+    return attr.sharedSizeBytes;
 }
 
-template <int THREADS, int NDW, bool SINGLE>
+// Image region of a decoder launch: the most that lets the most workgroups of `threads` lanes
+// share a CU (32 waves per CU, LDS blocks of 512 bytes) next to the table and the kernel's
+// static LDS, capped at what the largest possible tile needs; at least min(need, 16 * threads
+// + 48) bytes, else 0 (the shape does not fit).  Tiles whose symbols exceed the region take the
+// decoder's global-store path.
+static uint32_t dec_image_bytes(uint32_t threads, uint32_t table_words, uint32_t max_symbols,
+                                size_t static_bytes) {
+    const uint32_t need = dec_image_need(threads, max_symbols);
+    const uint32_t minimum = std::min(need, 16u * threads + 48u);
+    const size_t fixed = static_bytes + 4ull * dec_image_offset(table_words);
+    for (uint32_t wgs = 32u / (threads / 64u); wgs >= 1; wgs--) {
+        const size_t per_wg = (kMaxLds / wgs) & ~(size_t)511;
+        if (per_wg < fixed + minimum) continue;
+        const size_t budget = (per_wg - fixed) & ~(size_t)15;
+        return (uint32_t)std::min<size_t>(need, budget);
+    }
+    return 0;
+}
+
+template <int THREADS>
+static uint32_t dec_shape_image(uint32_t table_words, uint32_t max_symbols) {
+    size_t st = static_lds(dec_kernel<THREADS, true>);
+    st = std::max(st, static_lds(dec_kernel<THREADS, false>));
+    st = std::max(st, static_lds(dec_batch_kernel<THREADS>));
+    return dec_image_bytes(THREADS, table_words, max_symbols, st);
+}
+
+static uint32_t dec_shape_image(uint32_t threads, uint32_t table_words, uint32_t max_symbols) {
+    switch (threads) {
+    case 256: return dec_shape_image<256>(table_words, max_symbols);
+    case 512: return dec_shape_image<512>(table_words, max_symbols);
+    case 1024: return dec_shape_image<1024>(table_words, max_symbols);
+    default: return 0;
+    }
+}
+
+template <int THREADS, bool SINGLE>
 static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
                                 const hgap_dectable_info &info, uint32_t *counter,
                                 const hgap_launch_config &c, hipStream_t s) {
-    const uint32_t table_words = info.l1_entries + info.l2_entries;
-    const uint32_t slot_offset = (table_words + 3u) & ~3u;
-    const size_t lds = 4ull * slot_offset + dec_stage_bytes(THREADS, NDW);
-    auto kern = dec_kernel<THREADS, NDW, SINGLE>;
+    const uint32_t table_words = info.bytes / 4u;
+    const uint32_t image = dec_shape_image<THREADS>(table_words, info.max_symbols_per_segment);
+    if (image == 0) return HGAP_ERR_UNSUPPORTED;
+    const uint32_t image_offset = dec_image_offset(table_words);
+    const size_t lds = 4ull * image_offset + image;
+    auto kern = dec_kernel<THREADS, SINGLE>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
     DecTable t{info.prefix_bits, info.l1_entries};
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
-                       slot_offset, counter, window_per_lane(c));
+                       image_offset, image, counter, window_per_lane(c));
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
@@ -1237,18 +1343,8 @@ template <int THREADS>
 static hgap_status launch_dec_threads(const DecBuffer &b, const uint32_t *table,
                                       const hgap_dectable_info &info, uint32_t *counter,
                                       const hgap_launch_config &c, hipStream_t s) {
-    const bool single = info.l2_entries == 0;
-    switch (dec_slot_words(info.max_symbols_per_segment)) {
-    case 11:
-        return single ? launch_dec_t<THREADS, 11, true>(b, table, info, counter, c, s)
-                      : launch_dec_t<THREADS, 11, false>(b, table, info, counter, c, s);
-    case 17:
-        return single ? launch_dec_t<THREADS, 17, true>(b, table, info, counter, c, s)
-                      : launch_dec_t<THREADS, 17, false>(b, table, info, counter, c, s);
-    default:
-        return single ? launch_dec_t<THREADS, 33, true>(b, table, info, counter, c, s)
-                      : launch_dec_t<THREADS, 33, false>(b, table, info, counter, c, s);
-    }
+    return info.l2_entries == 0 ? launch_dec_t<THREADS, true>(b, table, info, counter, c, s)
+                                : launch_dec_t<THREADS, false>(b, table, info, counter, c, s);
 }
 
 static hgap_status launch_dec(const DecBuffer &b, const uint32_t *table,
@@ -1312,33 +1408,23 @@ static hgap_status launch_enc_batch(const EncJobDev *jobs, uint32_t count, uint3
     return HGAP_ERR_UNSUPPORTED;
 }
 
-template <int THREADS, int NDW>
-static hgap_status launch_dec_batch_t(const DecJobDev *jobs, uint32_t count, uint32_t total,
-                                      uint32_t max_table_words, uint32_t *ws,
-                                      const hgap_launch_config &c, hipStream_t s) {
-    const uint32_t slot_offset = (max_table_words + 3u) & ~3u;
-    const size_t lds = 4ull * slot_offset + dec_stage_bytes(THREADS, NDW);
-    auto kern = dec_batch_kernel<THREADS, NDW>;
-    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
-    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total, slot_offset,
-                       ws, reinterpret_cast<uint64_t *>(ws + 4), window_per_lane(c));
-    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
-}
-
 template <int THREADS>
 static hgap_status launch_dec_batch_threads(const DecJobDev *jobs, uint32_t count,
                                             uint32_t total, uint32_t max_table_words,
                                             uint32_t max_symbols, uint32_t *ws,
                                             const hgap_launch_config &c, hipStream_t s) {
-    switch (dec_slot_words(max_symbols)) {
-    case 11: return launch_dec_batch_t<THREADS, 11>(jobs, count, total, max_table_words, ws, c, s);
-    case 17: return launch_dec_batch_t<THREADS, 17>(jobs, count, total, max_table_words, ws, c, s);
-    default: return launch_dec_batch_t<THREADS, 33>(jobs, count, total, max_table_words, ws, c, s);
-    }
+    const uint32_t image = dec_shape_image<THREADS>(max_table_words, max_symbols);
+    if (image == 0) return HGAP_ERR_UNSUPPORTED;
+    const uint32_t image_offset = dec_image_offset(max_table_words);
+    const size_t lds = 4ull * image_offset + image;
+    auto kern = dec_batch_kernel<THREADS>;
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total, image_offset,
+                       image, ws, reinterpret_cast<uint64_t *>(ws + 4), window_per_lane(c));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
-// Whether static + `dyn_bytes` of dynamic LDS fit the CU (HIP's error state left clear).
 template <typename K>
 static bool lds_fits(K kernel, size_t dyn_bytes) {
     hipFuncAttributes attr;
@@ -1367,31 +1453,8 @@ static bool enc_fits(const hgap_launch_config &c, uint32_t max_len) {
     return false;
 }
 
-template <int THREADS, int NDW>
-static bool dec_shape_fits(uint32_t table_words) {
-    const size_t lds = 4ull * ((table_words + 3u) & ~3u) + dec_stage_bytes(THREADS, NDW);
-    return lds_fits(dec_kernel<THREADS, NDW, true>, lds) &&
-           lds_fits(dec_kernel<THREADS, NDW, false>, lds) &&
-           lds_fits(dec_batch_kernel<THREADS, NDW>, lds);
-}
-
-template <int THREADS>
-static bool dec_threads_fit(const hgap_dectable_info &info) {
-    const uint32_t tw = info.l1_entries + info.l2_entries;
-    switch (dec_slot_words(info.max_symbols_per_segment)) {
-    case 11: return dec_shape_fits<THREADS, 11>(tw);
-    case 17: return dec_shape_fits<THREADS, 17>(tw);
-    default: return dec_shape_fits<THREADS, 33>(tw);
-    }
-}
-
 static bool dec_fits(const hgap_launch_config &c, const hgap_dectable_info &info) {
-    switch (c.dec_threads) {
-    case 256: return dec_threads_fit<256>(info);
-    case 512: return dec_threads_fit<512>(info);
-    case 1024: return dec_threads_fit<1024>(info);
-    default: return false;
-    }
+    return dec_shape_image(c.dec_threads, info.bytes / 4u, info.max_symbols_per_segment) > 0;
 }
 
 }  // namespace hgap
@@ -1409,7 +1472,10 @@ hgap_status hgap_config_supported(const hgap_launch_config *cfg, uint32_t max_le
     return HGAP_OK;
 }
 
-uint32_t hgap_decode_slot_words(uint32_t max_symbols) { return dec_slot_words(max_symbols); }
+uint32_t hgap_decode_image_bytes(uint32_t dec_threads, uint32_t table_words,
+                                 uint32_t max_symbols) {
+    return dec_shape_image(dec_threads, table_words, max_symbols);
+}
 
 
 int hgap_device_cu_count(void) { return device_cus(); }
@@ -1526,7 +1592,8 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
         return HGAP_ERR_INVALID_ARG;
     if ((uint64_t)gap_elements != ((uint64_t)payload_words + 3) / 4) return HGAP_ERR_SIZE;
     if (info->symbol_count == 0 || info->prefix_bits < 1 || info->prefix_bits > 16 ||
-        info->l1_entries != (1u << info->prefix_bits) || info->min_length < 1)
+        info->l1_entries != (1u << info->prefix_bits) || info->min_length < 1 ||
+        info->bytes != 4u * (info->l1_entries + info->l2_entries + kDecLengthWords))
         return HGAP_ERR_CODEBOOK;
     if (workspace_bytes < hgap_decode_workspace_bytes(gap_elements)) return HGAP_ERR_WORKSPACE;
     const uint64_t tiles = ((uint64_t)gap_elements + c.dec_threads - 1) / c.dec_threads;
@@ -1660,7 +1727,8 @@ hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32
     if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
     if (count == 0 || total_tiles == 0) return HGAP_OK;
     if (!d_jobs || !d_workspace || !aligned16(d_workspace)) return HGAP_ERR_INVALID_ARG;
-    if (max_symbols < 1 || max_symbols > HGAP_SEGMENT_BITS || max_table_words < 2)
+    if (max_symbols < 1 || max_symbols > HGAP_SEGMENT_BITS ||
+        max_table_words < 2 + kDecLengthWords)
         return HGAP_ERR_CODEBOOK;
     if (workspace_bytes < hgap_batch_workspace_bytes(total_tiles)) return HGAP_ERR_WORKSPACE;
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -331,18 +331,15 @@ def support_masks(cb_arr, count, max_lens, threads=0):
     for p in tuner.PREFIX_BITS:
         iv = dectable_infos(cb_arr, count, p, threads)
         ms = iv["max_symbols_per_segment"].astype(np.int64)
-        um = np.unique(ms)
-        slot = np.array([lib.hgap_decode_slot_words(int(m)) for m in um])[np.searchsorted(um, ms)]
-        tw = iv["l1_entries"].astype(np.int64) + iv["l2_entries"]
-        # the check depends on the table's words and the slot size only: one query per pair
-        keys, first, inv = np.unique(np.stack([tw, slot], 1), axis=0, return_index=True,
-                                     return_inverse=True)
+        nbytes = iv["bytes"].astype(np.int64)
+        # the check depends on the table's size and the segment density only: one query per pair
+        keys, inv = np.unique(np.stack([nbytes, ms], 1), axis=0, return_inverse=True)
         cols = [j for j, c in enumerate(DEC_CANDIDATES) if c["prefix_bits"] == p]
         sup = np.zeros((len(keys), len(cols)), bool)
         for u in range(len(keys)):
             info = DectableInfo()
-            info.l1_entries, info.l2_entries = int(keys[u, 0]), 0
-            info.max_symbols_per_segment = int(ms[first[u]])
+            info.bytes = int(keys[u, 0])
+            info.max_symbols_per_segment = int(keys[u, 1])
             sup[u] = [_supported(lib, DEC_CANDIDATES[j], 0, info) for j in cols]
         dec[:, cols] = sup[inv.reshape(-1)]
     return enc, dec

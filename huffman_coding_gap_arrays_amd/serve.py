@@ -164,8 +164,7 @@ class _Batch:
             table = s.tables.get(r.compressed.codebook, int(cfg.prefix_bits))
             out = torch.empty(_up(16, r.n + 16), dtype=torch.uint8, device=s.device)
             r.result = out
-            slot = lib.hgap_decode_slot_words(table.info.max_symbols_per_segment)
-            key = (cfg.dec_threads, cfg.lookback_window, slot)
+            key = (cfg.dec_threads, cfg.lookback_window)
             groups.setdefault(key, (cfg, []))[1].append((r, table, out))
             self.keep.append(table)
         for cfg, members in groups.values():

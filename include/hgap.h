@@ -68,7 +68,8 @@ typedef struct hgap_dectable_info {
     uint32_t prefix_bits;   /* first-level index width */
     uint32_t l1_entries;    /* 1 << prefix_bits */
     uint32_t l2_entries;    /* second-level entries (0 when one level covers all codes) */
-    uint32_t bytes;         /* image size: 4 * (l1_entries + l2_entries) */
+    uint32_t bytes;         /* image size: 4 * (l1_entries + l2_entries + 64); the last 64
+                             * words hold the code length of each byte value */
     uint32_t min_length;    /* shortest code length */
     uint32_t max_length;    /* longest code length */
     uint32_t max_symbols_per_segment; /* bound on codewords starting in one 128-bit segment */
@@ -252,9 +253,12 @@ hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32
  * when it does not.  No counterpart in the reference (its launch shape is fixed). */
 hgap_status hgap_config_supported(const hgap_launch_config *cfg, uint32_t max_len,
                                   const hgap_dectable_info *info);
-/* LDS words of one decoder lane's output slot for tables whose segments hold at most
- * `max_symbols` codewords; batched decode launches group buffers of one slot size. */
-uint32_t hgap_decode_slot_words(uint32_t max_symbols);
+/* LDS bytes of the decoder's per-tile output image for `dec_threads` segments per tile, a
+ * table image of `table_words` words and segments of at most `max_symbols` codewords (0: the
+ * shape does not fit).  Sized for occupancy; tiles with more symbols than fit take a slower
+ * global-store path.  No counterpart in the reference. */
+uint32_t hgap_decode_image_bytes(uint32_t dec_threads, uint32_t table_words,
+                                 uint32_t max_symbols);
 
 /* ---- launch-shape predictor network ---------------------------------------------------------
  * A two-layer perceptron y = W2 relu(W1 x + b1) + b2 in fp32, trained by full-batch Adam on a

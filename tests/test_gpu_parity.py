@@ -130,10 +130,17 @@ def test_decode_prefix_splits(codec, orc, prefix):
     table = codec.DeviceTable(enc.codebook, prefix_bits=prefix)
     info = table.info
     assert (info.l2_entries == 0) == (prefix >= info.max_length)
-    ndw = codec._lib().hgap_decode_slot_words(info.max_symbols_per_segment)
-    # the default entry point falls back to 256-segment tiles before refusing
-    lds = 4 * ((info.bytes // 4 + 3) // 4 * 4) + 4 * (4 * 256 + 8 + 256 * ndw + 12)
-    if lds > 160 * 1024 - 64:
+    # the default entry point falls back to 256-segment tiles before refusing: refused when
+    # the table leaves less than 16 bytes per segment (+ 48) of LDS for the output image
+    image = codec._lib().hgap_decode_image_bytes(256, info.bytes // 4,
+                                                 info.max_symbols_per_segment)
+    need = (48 + 256 * info.max_symbols_per_segment + 15) // 16 * 16
+    fixed = 4 * ((info.bytes // 4 + 3) // 4 * 4) + min(need, 16 * 256 + 48)
+    if fixed <= 160 * 1024 - 256:  # static LDS of the kernels is under 256 bytes
+        assert 0 < image <= need and image % 16 == 0
+    elif fixed > 160 * 1024:
+        assert image == 0
+    if image == 0:
         with pytest.raises(codec.HgapError) as err:
             codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
         assert err.value.status == 8

@@ -102,12 +102,15 @@ def test_codebook_from_order_rejects_invalid(capi):
 
 
 def walk_native_table(image, info, payload, gap, W, G, n, check_second=True):
-    """Decodes with a libhgap table image following the kernel's rules in Python: segment starts
-    from the gap array; every step starts a codeword before the segment limit and keeps the
-    second codeword of a two-codeword entry only when it also starts before the limit;
-    two-level lookup for long codes.  check_second=False plants the mistake of keeping it
-    always."""
+    """Decodes with a libhgap table image in Python, by the format's rules: segment starts from
+    the gap array; every step starts a codeword before the segment limit; a direct entry is
+    sym1 | bits << 8 | sym2 << 16 | symbols << 24; a two-codeword entry whose second codeword
+    (length from the image's trailing length block) starts at or past the limit keeps only its
+    first; pointer entries (bit 31) lead to the second level.  check_second=False plants the
+    mistake of keeping the second codeword always."""
     P = info.prefix_bits
+    lengths = image[info.l1_entries + info.l2_entries:].view(np.uint8)
+    assert image.size == info.l1_entries + info.l2_entries + 64
     bits = np.unpackbits(payload.astype(">u4").view(np.uint8))
     bits = np.concatenate([bits, np.zeros(64, np.uint8)])
 
@@ -126,12 +129,15 @@ def walk_native_table(image, info, payload, gap, W, G, n, check_second=True):
         end = min(128 * (s + 1), 32 * W)
         while p < end:
             e = lookup(p)
-            len1 = (e >> 24) & 31
-            two = bool((e >> 13) & 1) and (p + len1 < end or not check_second)
+            assert e >> 31 == 0 and (e >> 24) in (1, 2)
+            used = (e >> 8) & 0xFF
             out.append(e & 0xFF)
-            if two:
-                out.append((e >> 16) & 0xFF)
-            p += ((e >> 8) & 31) if two else len1
+            if e >> 24 == 2:
+                len2 = int(lengths[(e >> 16) & 0xFF])
+                assert 1 <= len2 < used
+                if p + used - len2 < end or not check_second:
+                    out.append((e >> 16) & 0xFF)
+            p += used
     return np.array(out[:n], np.uint8)
 
 
@@ -167,6 +173,9 @@ def test_dectable_decodes_oracle_streams(capi, orc, prefix, dist):
     pay, gap = orc.encode(d, ocb, sz)
     out = walk_native_table(img, info, pay, gap, sz["payload_words"], sz["gap_elements"], d.size)
     assert (out == d).all()
+    # trailing block: the oracle's code length of every byte value (0 when absent)
+    tail = img[info.l1_entries + info.l2_entries:].view(np.uint8)
+    assert (tail == np.asarray(ocb["length"], np.uint8)).all()
     assert info.max_symbols_per_segment == -(-128 // info.min_length)
     if dist == "zipf" and prefix in (0, 11):
         # keeping a second codeword that starts past the segment limit duplicates symbols
@@ -225,6 +234,9 @@ def test_device_entry_points_validate_before_launch(capi):
     p = ctypes.c_void_p(16)
     assert capi.hgap_decode(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 4096, None) == 3
     info.symbol_count, info.prefix_bits, info.l1_entries, info.min_length = 2, 1, 2, 1
+    info.bytes = 4 * 2  # no trailing length block: not an image of this library
+    assert capi.hgap_decode(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 4096, None) == 3
+    info.bytes = 4 * (2 + 64)
     assert capi.hgap_decode(p, 8, p, 3, p, ctypes.byref(info), p, 10, p, 4096, None) == 2
     assert capi.hgap_decode(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 8, None) == 4
     assert capi.hgap_histogram(ctypes.c_void_p(3), 10, ctypes.c_void_p(16), None) == 1
