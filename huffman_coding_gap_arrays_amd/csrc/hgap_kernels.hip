@@ -5,9 +5,13 @@
 //   enc_kernel      persistent encoder: per-thread code lengths, workgroup scan, single-pass
 //                   decoupled look-back across tiles (64-bit status words in HBM), bit-packing
 //                   into an LDS tile with gap nibbles emitted on the fly, 128-bit stores
-//   dec_kernel      persistent decoder: two-level table in LDS, one lane per 128-bit segment
-//                   starting at its gap offset, symbol-count scan + look-back, LDS compaction
-//                   and 128-bit stores
+//   dec_count_kernel, dec_offsets_kernel, dec_write_kernel
+//                   decoder in three launches: per-segment codeword counts (one lane per
+//                   128-bit segment from its gap offset, two-level table in LDS), per-tile output
+//                   offsets, then a second decode of each segment into an LDS image of the
+//                   tile's output and 128-bit stores
+//   dec_batch_kernel persistent decoder of many buffers in one launch (single pass: counts,
+//                   scan, look-back across each buffer's tiles, second decode, stores)
 //
 // Reference citations are file:line relative to the reference repository root.
 #include <hip/hip_runtime.h>
@@ -645,6 +649,33 @@ struct DecSeg {
     uint32_t p0, limit;
 };
 
+// The lane's segment from its staged words: called by all 64 lanes of the wave (the word after
+// the segment comes from the successor lane; lane 63 loaded it).  Segments past the stream get
+// limit 0 (no codewords).
+__device__ __forceinline__ DecSeg dec_segment(const DecBuffer &buf, const DecStage &st,
+                                              uint32_t seg) {
+    DecSeg g;
+    g.w0 = st.q[0];
+    g.w1 = st.q[1];
+    g.w2 = st.q[2];
+    g.w3 = st.q[3];
+    g.w4 = __builtin_amdgcn_ds_bpermute(4 * ((lane_id() + 1) & 63u), (int)st.q[0]);
+    if (lane_id() == 63) g.w4 = st.extra;
+    g.p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
+    const uint64_t avail =
+        seg < buf.gap_elements ? 32ull * buf.payload_words - 128ull * seg : 0ull;
+    g.limit = avail < 128 ? (uint32_t)avail : 128u;
+    return g;
+}
+
+// the lane's first symbol (its segment's first codeword)
+template <bool SINGLE>
+__device__ __forceinline__ uint8_t dec_first(const DecSeg &g, const uint32_t *s_tab, uint32_t P,
+                                             uint32_t l1) {
+    const uint64_t bb = ((((uint64_t)g.w0) << 32) | g.w1) << g.p0;
+    return (uint8_t)dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);
+}
+
 // walk 1: the number of codewords starting in [p0, limit)
 template <bool SINGLE>
 __device__ __forceinline__ uint32_t dec_count(const DecSeg &g, const uint32_t *s_tab,
@@ -734,24 +765,11 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
                                          DecTable t, const uint32_t *s_tab,
                                          const uint8_t *s_len, uint8_t *s_img,
                                          uint32_t image_bytes, uint32_t *s_wave,
-                                         uint64_t *s_base, uint32_t wpl, NextTile &next,
-                                         DecStage &next_stage) {
+                                         uint64_t *s_base, uint32_t wpl) {
     const uint32_t P = t.prefix_bits;
     const uint32_t seg = tile * THREADS + threadIdx.x;
     const bool live = seg < buf.gap_elements;
-    DecSeg g;
-    g.w0 = st.q[0];
-    g.w1 = st.q[1];
-    g.w2 = st.q[2];
-    g.w3 = st.q[3];
-    // the word after this segment: the successor lane's first word (lane 63: loaded)
-    g.w4 = __builtin_amdgcn_ds_bpermute(4 * ((lane_id() + 1) & 63u), (int)st.q[0]);
-    if (lane_id() == 63) g.w4 = st.extra;
-    g.p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
-    {
-        const uint64_t avail = live ? 32ull * buf.payload_words - 128ull * seg : 0ull;
-        g.limit = avail < 128 ? (uint32_t)avail : 128u;
-    }
+    const DecSeg g = dec_segment(buf, st, seg);
 
     // 1. count; 2. scan, publish the tile's count and start the look-back
     const uint32_t cnt = live ? dec_count<SINGLE>(g, s_tab, s_len, P, t.l1_entries) : 0u;
@@ -764,27 +782,14 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
         // 3. decode into the image while the look-back loads are in flight
         if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, s_img + 16 + excl);
         __syncthreads();
-        if (cnt) {
-            const uint32_t first =
-                dec_lookup<SINGLE>(s_tab, (uint32_t)(((((uint64_t)g.w0) << 32) | g.w1) << g.p0 >> 32),
-                                   P, t.l1_entries);
-            s_img[16 + excl] = (uint8_t)first;
-        }
-        // 4. the tile's output offset, and the workgroup's next tile (claimed after this
-        // tile's look-back, so tiles are claimed in the order their predecessors resolve)
+        if (cnt) s_img[16 + excl] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+        // 4. the tile's output offset
         if (threadIdx.x < 64) {
             const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
-            if (threadIdx.x == 0) {
-                *s_base = O;
-                if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
-            }
+            if (threadIdx.x == 0) *s_base = O;
         }
         __syncthreads();
         const uint64_t O = *s_base;
-        if (next.counter) {
-            next.tile = *next.s_ticket;
-            next_stage = dec_fetch<THREADS>(buf, next.tile);
-        }
 
         // 5. store owned bytes [O, O + tile_syms) (capped at n) in aligned 16-byte chunks.
         // Chunk c starts at output byte (O & ~15) + 16c = image byte 16 + 16c - (O & 15): the
@@ -846,17 +851,10 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
         // each lane's bytes to global memory
         if (threadIdx.x < 64) {
             const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
-            if (threadIdx.x == 0) {
-                *s_base = O;
-                if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
-            }
+            if (threadIdx.x == 0) *s_base = O;
         }
         __syncthreads();
         const uint64_t O = *s_base;
-        if (next.counter) {
-            next.tile = *next.s_ticket;
-            next_stage = dec_fetch<THREADS>(buf, next.tile);
-        }
         if (cnt) {
             DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
             dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
@@ -874,32 +872,136 @@ __host__ __device__ constexpr uint32_t dec_image_need(uint32_t threads, uint32_t
     return (16u + threads * max_symbols + 32u + 15u) & ~15u;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Single-buffer decode in three launches, with no waiting between tiles:
+//   dec_count_kernel    walk 1 of every segment; its codeword count, one byte per segment
+//   dec_offsets_kernel  each tile's symbol total from those bytes, scanned into the tile's
+//                       output offset (blocks of 256 tiles chained by the look-back)
+//   dec_write_kernel    per tile: scan of its segment counts, walk 2 into an LDS image aligned
+//                       like the output (image byte j = output byte (O & ~15) + j), aligned
+//                       128-bit stores
+// The payload is read twice; in exchange no tile waits on another, and the persistent
+// workgroups take tiles blockIdx.x, blockIdx.x + gridDim.x, ... without a ticket, loading the
+// next tile's inputs into registers one tile ahead.
 template <int THREADS, bool SINGLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
-void dec_kernel(DecBuffer buf, DecTable t,
-                                                      const uint32_t *__restrict__ table,
-                                                      uint32_t table_words, uint32_t image_offset,
-                                                      uint32_t image_bytes,
-                                                      uint32_t *__restrict__ counter,
-                                                      uint32_t wpl) {
+void dec_count_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ table,
+                      uint32_t table_words, uint8_t *__restrict__ seg_counts) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
-    __shared__ uint32_t s_wave[THREADS / 64];
-    __shared__ uint64_t s_base;
-    __shared__ uint32_t s_tile;
     uint32_t *s_tab = s_dyn;
     const uint8_t *s_len = reinterpret_cast<const uint8_t *>(s_tab + table_words - kDecLengthWords);
-    uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);  // 16-byte aligned
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
-    NextTile next{counter, &s_tile, s_tile, nullptr};
-    DecStage st = dec_fetch<THREADS>(buf, next.tile);
-    while (next.tile < buf.num_tiles) {
-        const uint32_t cur = next.tile;
-        DecStage sn;
-        dec_tile<THREADS, SINGLE>(buf, cur, st, t, s_tab, s_len, s_img, image_bytes, s_wave,
-                                  &s_base, wpl, next, sn);
+    uint32_t tile = blockIdx.x;
+    DecStage st = dec_fetch<THREADS>(buf, tile);
+    while (tile < buf.num_tiles) {
+        const uint32_t nt = tile + gridDim.x;
+        const DecStage sn = dec_fetch<THREADS>(buf, nt);
+        const uint32_t seg = tile * THREADS + threadIdx.x;
+        const DecSeg g = dec_segment(buf, st, seg);
+        const uint32_t cnt =
+            seg < buf.gap_elements ? dec_count<SINGLE>(g, s_tab, s_len, t.prefix_bits, t.l1_entries)
+                                   : 0u;
+        seg_counts[seg] = (uint8_t)cnt;  // <= 128 codewords start in a segment
         st = sn;
+        tile = nt;
+    }
+}
+
+constexpr uint32_t kOffsetThreads = 256;  // tiles per dec_offsets_kernel block
+
+template <int SEGS>
+__global__ __launch_bounds__(kOffsetThreads) void dec_offsets_kernel(
+    const uint8_t *__restrict__ seg_counts, uint32_t tiles, uint64_t *__restrict__ tile_offsets,
+    uint64_t *__restrict__ status, uint32_t *__restrict__ counter, uint32_t wpl) {
+    __shared__ uint32_t s_wave[kOffsetThreads / 64];
+    __shared__ uint32_t s_blk;
+    __shared__ uint64_t s_base;
+    if (threadIdx.x == 0) s_blk = atomicAdd(counter, 1u);  // blocks in claim order
+    __syncthreads();
+    const uint32_t blk = s_blk;
+    const uint32_t tile = blk * kOffsetThreads + threadIdx.x;
+    uint32_t sum = 0;
+    if (tile < tiles) {
+        const u32x4 *p = reinterpret_cast<const u32x4 *>(seg_counts + (uint64_t)tile * SEGS);
+#pragma unroll 8
+        for (int i = 0; i < SEGS / 16; i++) {
+            const u32x4 v = p[i];
+#pragma unroll
+            for (int k = 0; k < 4; k++) sum = __builtin_amdgcn_udot4(v[k], 0x01010101u, sum, false);
+        }
+    }
+    uint32_t total;  // <= 256 tiles x 1024 segments x 128 codewords < 2^32
+    const uint32_t excl = block_excl_sum<kOffsetThreads>(sum, s_wave, total);
+    if (threadIdx.x < 64) {
+        const uint64_t before = look_back(status, blk, total, wpl);
+        if (threadIdx.x == 0) s_base = before;
+    }
+    __syncthreads();
+    if (tile < tiles) tile_offsets[tile] = s_base + excl;
+}
+
+template <int THREADS, bool SINGLE>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void dec_write_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ table,
+                      uint32_t table_words, uint32_t image_offset, uint32_t image_bytes,
+                      const uint8_t *__restrict__ seg_counts,
+                      const uint64_t *__restrict__ tile_offsets) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    __shared__ uint32_t s_wave[THREADS / 64];
+    uint32_t *s_tab = s_dyn;
+    uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);  // 16-byte aligned
+    const uint32_t P = t.prefix_bits;
+    for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
+    __syncthreads();
+    uint32_t tile = blockIdx.x;
+    DecStage st = dec_fetch<THREADS>(buf, tile);
+    uint32_t cnt = tile < buf.num_tiles ? seg_counts[tile * THREADS + threadIdx.x] : 0u;
+    while (tile < buf.num_tiles) {
+        const uint32_t nt = tile + gridDim.x;
+        const DecStage sn = dec_fetch<THREADS>(buf, nt);
+        const uint32_t cn = nt < buf.num_tiles ? seg_counts[nt * THREADS + threadIdx.x] : 0u;
+        const uint64_t O = tile_offsets[tile];
+        const DecSeg g = dec_segment(buf, st, tile * THREADS + threadIdx.x);
+        uint32_t tile_syms;
+        const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
+        const uint32_t lead = (uint32_t)(O & 15u);
+        if (lead + tile_syms + 17u <= image_bytes) {  // uniform
+            // walk 2 into the image; after a barrier every lane writes its first symbol again
+            // over the stray byte its predecessor left there
+            const uint32_t a = lead + excl;
+            if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, s_img + a);
+            __syncthreads();
+            if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+            __syncthreads();
+            // output bytes [O, O + tile_syms) capped at n: 16-byte chunks of the image, byte
+            // stores in the two chunks shared with neighbouring tiles
+            const uint64_t base = O & ~15ull;
+            uint64_t hi = O + tile_syms;
+            if (hi > buf.n) hi = buf.n;
+            const uint32_t chunks = (lead + tile_syms + 15u) >> 4;
+            for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
+                const uint64_t gpos = base + 16ull * c;
+                if (gpos >= O && gpos + 16 <= hi) {
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
+                } else {
+                    for (int i = 0; i < 16; i++)
+                        if (gpos + i >= O && gpos + i < hi)
+                            buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+                }
+            }
+        } else {
+            // more symbols than the image holds: each lane stores its bytes to global memory
+            if (cnt) {
+                DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
+                dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
+            }
+            __syncthreads();  // every wave has read this tile's s_wave
+        }
+        st = sn;
+        cnt = cn;
+        tile = nt;
     }
 }
 
@@ -1111,12 +1213,10 @@ void dec_batch_kernel(
         buf.reserved = 0;
         const DecTable t{job.prefix_bits, job.l1_entries};
         const DecStage st = dec_fetch<THREADS>(buf, g - job.base);
-        NextTile none{nullptr, nullptr, 0u, nullptr};
-        DecStage unused;
         const uint8_t *s_len =
             reinterpret_cast<const uint8_t *>(s_tab + job.table_words - kDecLengthWords);
         dec_tile<THREADS, false>(buf, g - job.base, st, t, s_tab, s_len, s_img, image_bytes,
-                                 s_wave, &s_base, wpl, none, unused);
+                                 s_wave, &s_base, wpl);
     }
 }
 
@@ -1306,10 +1406,39 @@ static uint32_t dec_image_bytes(uint32_t threads, uint32_t table_words, uint32_t
 
 template <int THREADS>
 static uint32_t dec_shape_image(uint32_t table_words, uint32_t max_symbols) {
-    size_t st = static_lds(dec_kernel<THREADS, true>);
-    st = std::max(st, static_lds(dec_kernel<THREADS, false>));
+    size_t st = static_lds(dec_write_kernel<THREADS, true>);
+    st = std::max(st, static_lds(dec_write_kernel<THREADS, false>));
     st = std::max(st, static_lds(dec_batch_kernel<THREADS>));
     return dec_image_bytes(THREADS, table_words, max_symbols, st);
+}
+
+// Workspace of the single-buffer decoder, laid out for the smallest tile any shape uses:
+// the offsets kernel's block ticket (16 bytes), its look-back status words, one count byte per
+// segment (rounded up to whole 1024-segment tiles), one 64-bit output offset per tile.
+struct DecWork {
+    uint32_t *counter;
+    uint64_t *status;
+    uint8_t *seg_counts;
+    uint64_t *tile_offsets;
+};
+
+static uint64_t dec_max_tiles(uint64_t gap_elements) {
+    return (gap_elements + kMinDecThreads - 1) / kMinDecThreads;
+}
+static uint64_t dec_status_bytes(uint64_t gap_elements) {
+    const uint64_t blocks = (dec_max_tiles(gap_elements) + kOffsetThreads - 1) / kOffsetThreads;
+    return (8 * blocks + 15) & ~15ull;
+}
+static uint64_t dec_count_bytes(uint64_t gap_elements) { return (gap_elements + 1023) & ~1023ull; }
+
+static DecWork dec_work(void *ws, uint64_t gap_elements) {
+    uint8_t *p = static_cast<uint8_t *>(ws);
+    DecWork w;
+    w.counter = reinterpret_cast<uint32_t *>(p);
+    w.status = reinterpret_cast<uint64_t *>(p + 16);
+    w.seg_counts = p + 16 + dec_status_bytes(gap_elements);
+    w.tile_offsets = reinterpret_cast<uint64_t *>(w.seg_counts + dec_count_bytes(gap_elements));
+    return w;
 }
 
 static uint32_t dec_shape_image(uint32_t threads, uint32_t table_words, uint32_t max_symbols) {
@@ -1323,37 +1452,46 @@ static uint32_t dec_shape_image(uint32_t threads, uint32_t table_words, uint32_t
 
 template <int THREADS, bool SINGLE>
 static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
-                                const hgap_dectable_info &info, uint32_t *counter,
+                                const hgap_dectable_info &info, const DecWork &w,
                                 const hgap_launch_config &c, hipStream_t s) {
     const uint32_t table_words = info.bytes / 4u;
     const uint32_t image = dec_shape_image<THREADS>(table_words, info.max_symbols_per_segment);
     if (image == 0) return HGAP_ERR_UNSUPPORTED;
     const uint32_t image_offset = dec_image_offset(table_words);
-    const size_t lds = 4ull * image_offset + image;
-    auto kern = dec_kernel<THREADS, SINGLE>;
-    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
-    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
-    DecTable t{info.prefix_bits, info.l1_entries};
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, table, table_words,
-                       image_offset, image, counter, window_per_lane(c));
+    auto count = dec_count_kernel<THREADS, SINGLE>;
+    auto write = dec_write_kernel<THREADS, SINGLE>;
+    const size_t lds_count = 4ull * table_words;
+    const size_t lds_write = 4ull * image_offset + image;
+    if (!allow_dynamic_lds(count, lds_count) || !allow_dynamic_lds(write, lds_write))
+        return HGAP_ERR_UNSUPPORTED;
+    const DecTable t{info.prefix_bits, info.l1_entries};
+    hipLaunchKernelGGL(count, dim3(grid_for(count, THREADS, lds_count, c.blocks_per_cu, b.num_tiles)),
+                       dim3(THREADS), lds_count, s, b, t, table, table_words, w.seg_counts);
+    const unsigned blocks = (b.num_tiles + kOffsetThreads - 1) / kOffsetThreads;
+    hipLaunchKernelGGL(dec_offsets_kernel<THREADS>, dim3(blocks), dim3(kOffsetThreads), 0, s,
+                       w.seg_counts, b.num_tiles, w.tile_offsets, w.status, w.counter,
+                       window_per_lane(c));
+    hipLaunchKernelGGL(write, dim3(grid_for(write, THREADS, lds_write, c.blocks_per_cu, b.num_tiles)),
+                       dim3(THREADS), lds_write, s, b, t, table, table_words, image_offset, image,
+                       w.seg_counts, w.tile_offsets);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
 template <int THREADS>
 static hgap_status launch_dec_threads(const DecBuffer &b, const uint32_t *table,
-                                      const hgap_dectable_info &info, uint32_t *counter,
+                                      const hgap_dectable_info &info, const DecWork &w,
                                       const hgap_launch_config &c, hipStream_t s) {
-    return info.l2_entries == 0 ? launch_dec_t<THREADS, true>(b, table, info, counter, c, s)
-                                : launch_dec_t<THREADS, false>(b, table, info, counter, c, s);
+    return info.l2_entries == 0 ? launch_dec_t<THREADS, true>(b, table, info, w, c, s)
+                                : launch_dec_t<THREADS, false>(b, table, info, w, c, s);
 }
 
 static hgap_status launch_dec(const DecBuffer &b, const uint32_t *table,
-                              const hgap_dectable_info &info, uint32_t *counter,
+                              const hgap_dectable_info &info, const DecWork &w,
                               const hgap_launch_config &c, hipStream_t s) {
     switch (c.dec_threads) {
-    case 256: return launch_dec_threads<256>(b, table, info, counter, c, s);
-    case 512: return launch_dec_threads<512>(b, table, info, counter, c, s);
-    case 1024: return launch_dec_threads<1024>(b, table, info, counter, c, s);
+    case 256: return launch_dec_threads<256>(b, table, info, w, c, s);
+    case 512: return launch_dec_threads<512>(b, table, info, w, c, s);
+    case 1024: return launch_dec_threads<1024>(b, table, info, w, c, s);
     default: return HGAP_ERR_UNSUPPORTED;
     }
 }
@@ -1573,8 +1711,8 @@ hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
 }
 
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements) {
-    const uint64_t tiles = ((uint64_t)gap_elements + kMinDecThreads - 1) / kMinDecThreads;
-    return 16 + 8 * tiles;
+    return 16 + dec_status_bytes(gap_elements) + dec_count_bytes(gap_elements) +
+           8 * dec_max_tiles(gap_elements);
 }
 
 hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
@@ -1598,8 +1736,10 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     if (workspace_bytes < hgap_decode_workspace_bytes(gap_elements)) return HGAP_ERR_WORKSPACE;
     const uint64_t tiles = ((uint64_t)gap_elements + c.dec_threads - 1) / c.dec_threads;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
-    hgap_status st = launch_zero(ws, 4 + 2ull * tiles, nullptr, 0, s);
+    const DecWork w = dec_work(d_workspace, gap_elements);
+    // the offsets kernel's ticket and look-back status start at zero
+    const uint64_t blocks = (tiles + kOffsetThreads - 1) / kOffsetThreads;
+    hgap_status st = launch_zero(w.counter, 4 + 2 * blocks, nullptr, 0, s);
     if (st != HGAP_OK) return st;
     DecBuffer b;
     b.payload = d_payload;
@@ -1608,10 +1748,10 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     b.gap = d_gap;
     b.out = static_cast<uint8_t *>(d_out);
     b.n = n;
-    b.status = reinterpret_cast<uint64_t *>(ws + 4);
+    b.status = nullptr;
     b.num_tiles = (uint32_t)tiles;
     b.reserved = 0;
-    return launch_dec(b, static_cast<const uint32_t *>(d_table), *info, ws, c, s);
+    return launch_dec(b, static_cast<const uint32_t *>(d_table), *info, w, c, s);
 }
 
 // Default shape first; when table + staging do not fit in LDS, fewer segments per tile.
