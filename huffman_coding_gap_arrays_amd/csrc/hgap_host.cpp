@@ -127,7 +127,8 @@ void package_merge_lengths(const uint64_t *w, int k, uint8_t *len) {
 }
 
 // Direct decode-table entry: sym1 | bits << 8 | sym2 << 16 | symbols << 24 (one codeword:
-// bits = its length, symbols = 1; two codewords: bits = len1 + len2, symbols = 2).
+// bits = its length, symbols = 1; two codewords: bits = len1 + len2, symbols = 2).  The two
+// output bytes sit where byte stores of the low and high halves take them.
 static inline uint32_t single_entry(uint32_t sym, uint32_t len) {
     return sym | (len << 8) | (1u << 24);
 }

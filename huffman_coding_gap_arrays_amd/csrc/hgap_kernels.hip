@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "hgap.h"
 #include "hgap_internal.h"
@@ -676,68 +677,92 @@ __device__ __forceinline__ uint8_t dec_first(const DecSeg &g, const uint32_t *s_
     return (uint8_t)dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);
 }
 
+// Field access of direct table entries, as stored (sym1 | bits << 8 | sym2 << 16 | symbols << 24)
+// or rotated right by 8 (bits | sym2 << 8 | symbols << 16 | sym1 << 24, the count walk's copy of
+// a one-level table: the 64-bit shift then takes the entry itself as its amount).
+struct EntStored {
+    static __device__ __forceinline__ uint32_t bits(uint32_t e) { return (e >> 8) & 0xFFu; }
+    static __device__ __forceinline__ uint32_t symbols(uint32_t e) { return e >> 24; }
+    static __device__ __forceinline__ uint32_t sym2(uint32_t e) { return (e >> 16) & 0xFFu; }
+};
+struct EntRotated {
+    static __device__ __forceinline__ uint32_t bits(uint32_t e) { return e & 0xFFu; }
+    static __device__ __forceinline__ uint32_t symbols(uint32_t e) { return (e >> 16) & 0xFFu; }
+    static __device__ __forceinline__ uint32_t sym2(uint32_t e) { return (e >> 8) & 0xFFu; }
+};
+
+// Bit buffer refill, shared by both walks: `q` = fill - 32, so a refill is due when p > q and
+// shifts the next word in by p - q.
+#define HGAP_REFILL()                                                                         \
+    if (p > q) {                                                                              \
+        bb |= ((uint64_t)nx) << (p - q);                                                      \
+        q += 32u;                                                                             \
+        nx = nq;                                                                              \
+        nq = nq2;                                                                             \
+    }
+
 // walk 1: the number of codewords starting in [p0, limit)
-template <bool SINGLE>
+template <bool SINGLE, typename E>
 __device__ __forceinline__ uint32_t dec_count(const DecSeg &g, const uint32_t *s_tab,
                                               const uint8_t *s_len, uint32_t P, uint32_t l1) {
-    uint32_t p = g.p0, fill = 64, nx = g.w2, nq = g.w3, nq2 = g.w4;
+    uint32_t p = g.p0, q = 32, nx = g.w2, nq = g.w3, nq2 = g.w4;
     uint64_t bb = ((((uint64_t)g.w0) << 32) | g.w1) << p;
     uint32_t cnt = 0, last = 0;
 #define HGAP_COUNT_STEP()                                                                     \
     {                                                                                         \
         const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);            \
-        cnt += ent >> 24;                                                                     \
-        const uint32_t adv = (ent >> 8) & 0xFFu;                                              \
-        p += adv;                                                                             \
-        bb <<= adv;                                                                           \
+        cnt += E::symbols(ent);                                                               \
+        p += E::bits(ent);                                                                    \
+        bb <<= E::bits(ent) & 63u;                                                            \
         last = ent;                                                                           \
     }
     while (p < g.limit) {
         HGAP_COUNT_STEP()
         if (p < g.limit) HGAP_COUNT_STEP()
-        if (fill - p < 32u) {
-            bb |= ((uint64_t)nx) << (p + 32u - fill);
-            fill += 32u;
-            nx = nq;
-            nq = nq2;
-        }
+        HGAP_REFILL()
     }
 #undef HGAP_COUNT_STEP
     // undo the last step's second codeword when it starts at p - len2 >= limit
-    if ((last >> 24) == 2u && p - s_len[(last >> 16) & 0xFFu] >= g.limit) cnt--;
+    if (E::symbols(last) == 2u && p - s_len[E::sym2(last)] >= g.limit) cnt--;
     return cnt;
 }
 
-// walk 2: the same steps, storing two bytes per step at wp (LDS image or global output).  The
-// bytes a step stores past the lane's count are overwritten by its next step, except after
-// the last step: one stray byte just past the lane's range.
-template <bool SINGLE, typename Ptr>
+// walk 2: the same steps, storing the entry's two output bytes per step at the writer (LDS
+// image or global output).  The byte a step stores past the lane's count is overwritten by its
+// next step, except after the last step: one stray byte just past the lane's range.
+template <bool SINGLE, typename Writer>
 __device__ __forceinline__ void dec_write(const DecSeg &g, const uint32_t *s_tab, uint32_t P,
-                                          uint32_t l1, Ptr wp) {
-    uint32_t p = g.p0, fill = 64, nx = g.w2, nq = g.w3, nq2 = g.w4;
+                                          uint32_t l1, Writer wp) {
+    uint32_t p = g.p0, q = 32, nx = g.w2, nq = g.w3, nq2 = g.w4;
     uint64_t bb = ((((uint64_t)g.w0) << 32) | g.w1) << p;
 #define HGAP_WRITE_STEP()                                                                     \
     {                                                                                         \
         const uint32_t ent = dec_lookup<SINGLE>(s_tab, (uint32_t)(bb >> 32), P, l1);            \
-        wp[0] = (uint8_t)ent;                                                                 \
-        wp[1] = (uint8_t)(ent >> 16);                                                         \
-        wp += ent >> 24;                                                                      \
-        const uint32_t adv = (ent >> 8) & 0xFFu;                                              \
-        p += adv;                                                                             \
-        bb <<= adv;                                                                           \
+        wp.put2(ent);                                                                         \
+        wp.advance(EntStored::symbols(ent));                                                  \
+        p += EntStored::bits(ent);                                                            \
+        bb <<= EntStored::bits(ent) & 63u;                                                    \
     }
     while (p < g.limit) {
         HGAP_WRITE_STEP()
         if (p < g.limit) HGAP_WRITE_STEP()
-        if (fill - p < 32u) {
-            bb |= ((uint64_t)nx) << (p + 32u - fill);
-            fill += 32u;
-            nx = nq;
-            nq = nq2;
-        }
+        HGAP_REFILL()
     }
 #undef HGAP_WRITE_STEP
 }
+#undef HGAP_REFILL
+
+// LDS writer for walk 2: two byte stores, of the entry's bytes 0 and 2 (single-byte stores
+// take them from the data register without shifts; a 16-bit store at an odd address is much
+// slower on gfx950 LDS).
+struct DecLdsOut {
+    uint8_t *p;
+    __device__ __forceinline__ void put2(uint32_t ent) {
+        p[0] = (uint8_t)ent;
+        p[1] = (uint8_t)(ent >> 16);
+    }
+    __device__ __forceinline__ void advance(uint32_t d) { p += d; }
+};
 
 // Global-memory writer for walk 2 of a tile that skips the LDS image: stores symbol i of the
 // lane at out[o + i] when i < cnt and o + i < n (the stray byte is never stored).
@@ -748,16 +773,11 @@ struct DecGlobalOut {
     __device__ __forceinline__ void put(uint32_t k, uint8_t v) {
         if (i + k < cnt && o + i + k < n) out[o + i + k] = v;
     }
-    struct Ref {
-        DecGlobalOut *w;
-        uint32_t k;
-        __device__ __forceinline__ void operator=(uint8_t v) { w->put(k, v); }
-    };
-    __device__ __forceinline__ Ref operator[](uint32_t k) { return Ref{this, k}; }
-    __device__ __forceinline__ DecGlobalOut &operator+=(uint32_t d) {
-        i += d;
-        return *this;
+    __device__ __forceinline__ void put2(uint32_t ent) {
+        put(0, (uint8_t)ent);
+        put(1, (uint8_t)(ent >> 16));
     }
+    __device__ __forceinline__ void advance(uint32_t d) { i += d; }
 };
 
 template <int THREADS, bool SINGLE>
@@ -772,7 +792,7 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
     const DecSeg g = dec_segment(buf, st, seg);
 
     // 1. count; 2. scan, publish the tile's count and start the look-back
-    const uint32_t cnt = live ? dec_count<SINGLE>(g, s_tab, s_len, P, t.l1_entries) : 0u;
+    const uint32_t cnt = live ? dec_count<SINGLE, EntStored>(g, s_tab, s_len, P, t.l1_entries) : 0u;
     uint32_t tile_syms;
     const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
     LookBack lb;
@@ -780,7 +800,7 @@ __device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, co
 
     if (16u + tile_syms + 32u <= image_bytes) {  // uniform
         // 3. decode into the image while the look-back loads are in flight
-        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, s_img + 16 + excl);
+        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + 16 + excl});
         __syncthreads();
         if (cnt) s_img[16 + excl] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
         // 4. the tile's output offset
@@ -890,7 +910,13 @@ void dec_count_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     uint32_t *s_tab = s_dyn;
     const uint8_t *s_len = reinterpret_cast<const uint8_t *>(s_tab + table_words - kDecLengthWords);
-    for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
+    // one-level tables are stored rotated (EntRotated); the length block is copied as is
+    using E = typename std::conditional<SINGLE, EntRotated, EntStored>::type;
+    const uint32_t entries = table_words - kDecLengthWords;
+    for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) {
+        const uint32_t e = table[i];
+        s_tab[i] = SINGLE && i < entries ? __builtin_amdgcn_alignbit(e, e, 8) : e;
+    }
     __syncthreads();
     uint32_t tile = blockIdx.x;
     DecStage st = dec_fetch<THREADS>(buf, tile);
@@ -900,8 +926,9 @@ void dec_count_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
         const uint32_t seg = tile * THREADS + threadIdx.x;
         const DecSeg g = dec_segment(buf, st, seg);
         const uint32_t cnt =
-            seg < buf.gap_elements ? dec_count<SINGLE>(g, s_tab, s_len, t.prefix_bits, t.l1_entries)
-                                   : 0u;
+            seg < buf.gap_elements
+                ? dec_count<SINGLE, E>(g, s_tab, s_len, t.prefix_bits, t.l1_entries)
+                : 0u;
         seg_counts[seg] = (uint8_t)cnt;  // <= 128 codewords start in a segment
         st = sn;
         tile = nt;
@@ -970,7 +997,7 @@ void dec_write_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
             // walk 2 into the image; after a barrier every lane writes its first symbol again
             // over the stray byte its predecessor left there
             const uint32_t a = lead + excl;
-            if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, s_img + a);
+            if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
             __syncthreads();
             if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
             __syncthreads();
