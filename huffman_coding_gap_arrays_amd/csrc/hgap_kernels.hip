@@ -895,8 +895,9 @@ __host__ __device__ constexpr uint32_t dec_image_need(uint32_t threads, uint32_t
 // ---------------------------------------------------------------------------------------------
 // Single-buffer decode in three launches, with no waiting between tiles:
 //   dec_count_kernel    walk 1 of every segment; its codeword count, one byte per segment
-//   dec_offsets_kernel  each tile's symbol total from those bytes, scanned into the tile's
-//                       output offset (blocks of 256 tiles chained by the look-back)
+//   dec_offsets_kernel  each tile's symbol total (from per-wave sums the count kernel stores
+//                       beside the bytes), scanned into the tile's output offset (blocks of
+//                       1024 tiles chained by the look-back)
 //   dec_write_kernel    per tile: scan of its segment counts, walk 2 into an LDS image aligned
 //                       like the output (image byte j = output byte (O & ~15) + j), aligned
 //                       128-bit stores
@@ -906,7 +907,8 @@ __host__ __device__ constexpr uint32_t dec_image_need(uint32_t threads, uint32_t
 template <int THREADS, bool SINGLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void dec_count_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ table,
-                      uint32_t table_words, uint8_t *__restrict__ seg_counts) {
+                      uint32_t table_words, uint8_t *__restrict__ seg_counts,
+                      uint32_t *__restrict__ wave_sums) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     uint32_t *s_tab = s_dyn;
     const uint8_t *s_len = reinterpret_cast<const uint8_t *>(s_tab + table_words - kDecLengthWords);
@@ -930,42 +932,60 @@ void dec_count_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
                 ? dec_count<SINGLE, E>(g, s_tab, s_len, t.prefix_bits, t.l1_entries)
                 : 0u;
         seg_counts[seg] = (uint8_t)cnt;  // <= 128 codewords start in a segment
+        const uint32_t wsum = wave_incl_sum(cnt);
+        if (lane_id() == 63) wave_sums[seg >> 6] = wsum;
         st = sn;
         tile = nt;
     }
 }
 
-constexpr uint32_t kOffsetThreads = 256;  // tiles per dec_offsets_kernel block
+constexpr uint32_t kOffsetThreads = 256;
+constexpr uint32_t kOffsetTilesPerThread = 4;
+constexpr uint32_t kOffsetTiles = kOffsetThreads * kOffsetTilesPerThread;  // per block
 
+// Tile t's output offset: the sum of the wave sums of tiles before t.  A block covers 1024
+// tiles (4 consecutive per thread); blocks are chained by the look-back in claim order.
 template <int SEGS>
 __global__ __launch_bounds__(kOffsetThreads) void dec_offsets_kernel(
-    const uint8_t *__restrict__ seg_counts, uint32_t tiles, uint64_t *__restrict__ tile_offsets,
+    const uint32_t *__restrict__ wave_sums, uint32_t tiles, uint64_t *__restrict__ tile_offsets,
     uint64_t *__restrict__ status, uint32_t *__restrict__ counter, uint32_t wpl) {
+    constexpr uint32_t W = SEGS / 64;  // wave sums per tile (4, 8 or 16)
     __shared__ uint32_t s_wave[kOffsetThreads / 64];
     __shared__ uint32_t s_blk;
     __shared__ uint64_t s_base;
     if (threadIdx.x == 0) s_blk = atomicAdd(counter, 1u);  // blocks in claim order
     __syncthreads();
     const uint32_t blk = s_blk;
-    const uint32_t tile = blk * kOffsetThreads + threadIdx.x;
-    uint32_t sum = 0;
-    if (tile < tiles) {
-        const u32x4 *p = reinterpret_cast<const u32x4 *>(seg_counts + (uint64_t)tile * SEGS);
-#pragma unroll 8
-        for (int i = 0; i < SEGS / 16; i++) {
-            const u32x4 v = p[i];
+    const uint32_t tile0 = blk * kOffsetTiles + threadIdx.x * kOffsetTilesPerThread;
+    uint32_t sums[kOffsetTilesPerThread];
+    uint32_t mine = 0;
 #pragma unroll
-            for (int k = 0; k < 4; k++) sum = __builtin_amdgcn_udot4(v[k], 0x01010101u, sum, false);
+    for (uint32_t k = 0; k < kOffsetTilesPerThread; k++) {
+        uint32_t v = 0;
+        if (tile0 + k < tiles) {
+            const u32x4 *p = reinterpret_cast<const u32x4 *>(wave_sums + (uint64_t)(tile0 + k) * W);
+#pragma unroll
+            for (uint32_t i = 0; i < W / 4; i++) {
+                const u32x4 x = p[i];
+                v += x[0] + x[1] + x[2] + x[3];
+            }
         }
+        sums[k] = v;
+        mine += v;
     }
-    uint32_t total;  // <= 256 tiles x 1024 segments x 128 codewords < 2^32
-    const uint32_t excl = block_excl_sum<kOffsetThreads>(sum, s_wave, total);
+    uint32_t total;  // <= 1024 tiles x 1024 segments x 128 codewords = 2^27
+    const uint32_t excl = block_excl_sum<kOffsetThreads>(mine, s_wave, total);
     if (threadIdx.x < 64) {
         const uint64_t before = look_back(status, blk, total, wpl);
         if (threadIdx.x == 0) s_base = before;
     }
     __syncthreads();
-    if (tile < tiles) tile_offsets[tile] = s_base + excl;
+    uint64_t o = s_base + excl;
+#pragma unroll
+    for (uint32_t k = 0; k < kOffsetTilesPerThread; k++) {
+        if (tile0 + k < tiles) tile_offsets[tile0 + k] = o;
+        o += sums[k];
+    }
 }
 
 template <int THREADS, bool SINGLE>
@@ -1441,11 +1461,13 @@ static uint32_t dec_shape_image(uint32_t table_words, uint32_t max_symbols) {
 
 // Workspace of the single-buffer decoder, laid out for the smallest tile any shape uses:
 // the offsets kernel's block ticket (16 bytes), its look-back status words, one count byte per
-// segment (rounded up to whole 1024-segment tiles), one 64-bit output offset per tile.
+// segment (rounded up to whole 1024-segment tiles), one 32-bit sum per 64 of those segments,
+// one 64-bit output offset per tile.
 struct DecWork {
     uint32_t *counter;
     uint64_t *status;
     uint8_t *seg_counts;
+    uint32_t *wave_sums;
     uint64_t *tile_offsets;
 };
 
@@ -1453,7 +1475,7 @@ static uint64_t dec_max_tiles(uint64_t gap_elements) {
     return (gap_elements + kMinDecThreads - 1) / kMinDecThreads;
 }
 static uint64_t dec_status_bytes(uint64_t gap_elements) {
-    const uint64_t blocks = (dec_max_tiles(gap_elements) + kOffsetThreads - 1) / kOffsetThreads;
+    const uint64_t blocks = (dec_max_tiles(gap_elements) + kOffsetTiles - 1) / kOffsetTiles;
     return (8 * blocks + 15) & ~15ull;
 }
 static uint64_t dec_count_bytes(uint64_t gap_elements) { return (gap_elements + 1023) & ~1023ull; }
@@ -1464,7 +1486,9 @@ static DecWork dec_work(void *ws, uint64_t gap_elements) {
     w.counter = reinterpret_cast<uint32_t *>(p);
     w.status = reinterpret_cast<uint64_t *>(p + 16);
     w.seg_counts = p + 16 + dec_status_bytes(gap_elements);
-    w.tile_offsets = reinterpret_cast<uint64_t *>(w.seg_counts + dec_count_bytes(gap_elements));
+    w.wave_sums = reinterpret_cast<uint32_t *>(w.seg_counts + dec_count_bytes(gap_elements));
+    w.tile_offsets = reinterpret_cast<uint64_t *>(w.seg_counts + dec_count_bytes(gap_elements) +
+                                                  dec_count_bytes(gap_elements) / 16);
     return w;
 }
 
@@ -1493,10 +1517,11 @@ static hgap_status launch_dec_t(const DecBuffer &b, const uint32_t *table,
         return HGAP_ERR_UNSUPPORTED;
     const DecTable t{info.prefix_bits, info.l1_entries};
     hipLaunchKernelGGL(count, dim3(grid_for(count, THREADS, lds_count, c.blocks_per_cu, b.num_tiles)),
-                       dim3(THREADS), lds_count, s, b, t, table, table_words, w.seg_counts);
-    const unsigned blocks = (b.num_tiles + kOffsetThreads - 1) / kOffsetThreads;
+                       dim3(THREADS), lds_count, s, b, t, table, table_words, w.seg_counts,
+                       w.wave_sums);
+    const unsigned blocks = (b.num_tiles + kOffsetTiles - 1) / kOffsetTiles;
     hipLaunchKernelGGL(dec_offsets_kernel<THREADS>, dim3(blocks), dim3(kOffsetThreads), 0, s,
-                       w.seg_counts, b.num_tiles, w.tile_offsets, w.status, w.counter,
+                       w.wave_sums, b.num_tiles, w.tile_offsets, w.status, w.counter,
                        window_per_lane(c));
     hipLaunchKernelGGL(write, dim3(grid_for(write, THREADS, lds_write, c.blocks_per_cu, b.num_tiles)),
                        dim3(THREADS), lds_write, s, b, t, table, table_words, image_offset, image,
@@ -1739,7 +1764,7 @@ hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
 
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements) {
     return 16 + dec_status_bytes(gap_elements) + dec_count_bytes(gap_elements) +
-           8 * dec_max_tiles(gap_elements);
+           dec_count_bytes(gap_elements) / 16 + 8 * dec_max_tiles(gap_elements);
 }
 
 hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
@@ -1765,7 +1790,7 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     hipStream_t s = static_cast<hipStream_t>(stream);
     const DecWork w = dec_work(d_workspace, gap_elements);
     // the offsets kernel's ticket and look-back status start at zero
-    const uint64_t blocks = (tiles + kOffsetThreads - 1) / kOffsetThreads;
+    const uint64_t blocks = (tiles + kOffsetTiles - 1) / kOffsetTiles;
     hgap_status st = launch_zero(w.counter, 4 + 2 * blocks, nullptr, 0, s);
     if (st != HGAP_OK) return st;
     DecBuffer b;

@@ -243,8 +243,8 @@ def test_device_entry_points_validate_before_launch(capi):
     assert capi.hgap_encode_workspace_bytes(0) == 16
     assert capi.hgap_encode_workspace_bytes(8193) == 16 + 3 * 8  # smallest tile: 4 KiB
     # ticket + status word of one offsets block, one count byte per segment (whole 1024-segment
-    # tiles), one offset per 256-segment tile (the smallest tile)
-    assert capi.hgap_decode_workspace_bytes(257) == 16 + 16 + 1024 + 2 * 8
+    # tiles), one sum per 64 of those, one offset per 256-segment tile (the smallest tile)
+    assert capi.hgap_decode_workspace_bytes(257) == 16 + 16 + 1024 + 64 + 2 * 8
     # launch shapes outside the supported set are refused before anything is launched
     from huffman_coding_gap_arrays_amd._capi import LaunchConfig, default_config
 
