@@ -213,6 +213,7 @@ PROTOTYPES = {
     "hgap_batch_plan_encode": (_U32, [_P, _U32, ctypes.c_void_p]),
     "hgap_batch_plan_decode": (_U32, [_P, _U32, ctypes.c_void_p]),
     "hgap_batch_workspace_bytes": (_SZ, [_U32]),
+    "hgap_batch_decode_workspace_bytes": (_SZ, [_U32, ctypes.POINTER(LaunchConfig)]),
     "hgap_batch_encode": (_INT, [_P, _U32, _U32, _U32, _P, _P, _SZ, ctypes.c_void_p, _P]),
     "hgap_batch_decode": (_INT, [_P, _U32, _U32, _U32, _U32, _P, _SZ, ctypes.c_void_p, _P]),
     "hgap_synth_fill": (_INT, [_P, _U64, _U64, _U64, _P, _P, _P]),

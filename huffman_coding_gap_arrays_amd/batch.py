@@ -179,9 +179,12 @@ class BatchCodec:
     def sizes(self):
         return [self.size_array[i] for i in range(self.count)]
 
-    def _workspace(self, total):
-        return torch.empty(_up(16, _lib().hgap_batch_workspace_bytes(total)), dtype=torch.uint8,
-                           device=self.device)
+    def _workspace(self, total, dec_cfg=None):
+        """Encode workspace for `total` tiles, or the decode workspace of shape `dec_cfg`."""
+        lib = _lib()
+        nbytes = (lib.hgap_batch_workspace_bytes(total) if dec_cfg is None
+                  else lib.hgap_batch_decode_workspace_bytes(total, ctypes.byref(dec_cfg)))
+        return torch.empty(_up(16, nbytes), dtype=torch.uint8, device=self.device)
 
     def _plan_groups(self):
         lib = _lib()
@@ -234,7 +237,7 @@ class BatchCodec:
                                           count=len(idx), total=total,
                                           max_tw=int(max(2, tw[idx].max())),
                                           max_sym=int(max(1, ms[idx].max())),
-                                          ws=self._workspace(total), members=idx))
+                                          ws=self._workspace(total, cfg), members=idx))
 
     def encode(self, stream=None):
         lib = _lib()

@@ -17,7 +17,7 @@ constexpr uint32_t kDefaultPrefixBits = 11;
 // (tuned on the 1 GiB Zipf(1.1) buffer with huffman_coding_gap_arrays_amd/tuner.py)
 constexpr uint32_t kDefaultEncThreads = 512;
 constexpr uint32_t kDefaultEncBytesPerThread = 64;
-constexpr uint32_t kDefaultDecThreads = 512;
+constexpr uint32_t kDefaultDecThreads = 256;
 constexpr uint32_t kDefaultLookbackWindow = 64;
 // smallest tiles any configuration uses (they size the look-back workspaces)
 constexpr uint32_t kMinEncTileBytes = 256 * 16;

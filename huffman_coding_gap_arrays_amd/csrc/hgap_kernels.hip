@@ -10,8 +10,9 @@
 //                   128-bit segment from its gap offset, two-level table in LDS), per-tile output
 //                   offsets, then a second decode of each segment into an LDS image of the
 //                   tile's output and 128-bit stores
-//   dec_batch_kernel persistent decoder of many buffers in one launch (single pass: counts,
-//                   scan, look-back across each buffer's tiles, second decode, stores)
+//   dec_count_batch_kernel, dec_write_batch_kernel
+//                   the same decode over many buffers (each with its own table) in one launch
+//                   per step
 //
 // Reference citations are file:line relative to the reference repository root.
 #include <hip/hip_runtime.h>
@@ -780,110 +781,6 @@ struct DecGlobalOut {
     __device__ __forceinline__ void advance(uint32_t d) { i += d; }
 };
 
-template <int THREADS, bool SINGLE>
-__device__ __forceinline__ void dec_tile(const DecBuffer &buf, uint32_t tile, const DecStage &st,
-                                         DecTable t, const uint32_t *s_tab,
-                                         const uint8_t *s_len, uint8_t *s_img,
-                                         uint32_t image_bytes, uint32_t *s_wave,
-                                         uint64_t *s_base, uint32_t wpl) {
-    const uint32_t P = t.prefix_bits;
-    const uint32_t seg = tile * THREADS + threadIdx.x;
-    const bool live = seg < buf.gap_elements;
-    const DecSeg g = dec_segment(buf, st, seg);
-
-    // 1. count; 2. scan, publish the tile's count and start the look-back
-    const uint32_t cnt = live ? dec_count<SINGLE, EntStored>(g, s_tab, s_len, P, t.l1_entries) : 0u;
-    uint32_t tile_syms;
-    const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
-    LookBack lb;
-    if (threadIdx.x < 64) lb_start(buf.status, tile, tile_syms, wpl, lb);
-
-    if (16u + tile_syms + 32u <= image_bytes) {  // uniform
-        // 3. decode into the image while the look-back loads are in flight
-        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + 16 + excl});
-        __syncthreads();
-        if (cnt) s_img[16 + excl] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
-        // 4. the tile's output offset
-        if (threadIdx.x < 64) {
-            const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
-            if (threadIdx.x == 0) *s_base = O;
-        }
-        __syncthreads();
-        const uint64_t O = *s_base;
-
-        // 5. store owned bytes [O, O + tile_syms) (capped at n) in aligned 16-byte chunks.
-        // Chunk c starts at output byte (O & ~15) + 16c = image byte 16 + 16c - (O & 15): the
-        // 16 bytes at shift sh = 16 - (O & 15) into image chunks c and c + 1 (sh is the same
-        // for every chunk).
-        const uint64_t lo = O;
-        uint64_t hi = O + tile_syms;
-        if (hi > buf.n) hi = buf.n;
-        if (hi > lo) {
-            const uint64_t base = O & ~15ull;
-            const uint32_t chunks = (uint32_t)((hi - base + 15) >> 4);
-            const uint32_t sh = 16u - (uint32_t)(O & 15u);  // 1..16
-            const uint32_t qw = sh >> 2, rb = sh & 3u;        // whole words (0..4), then bytes
-            for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
-                const u32x4 x0 = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
-                const u32x4 x1 = *reinterpret_cast<const u32x4 *>(s_img + 16 * c + 16);
-                const uint32_t x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-                uint32_t w[5];
-                // words qw..qw+4 of the 32-byte window (qw is uniform: a scalar branch)
-                switch (qw) {
-                case 0:
-#pragma unroll
-                    for (int k = 0; k < 5; k++) w[k] = x[k];
-                    break;
-                case 1:
-#pragma unroll
-                    for (int k = 0; k < 5; k++) w[k] = x[1 + k];
-                    break;
-                case 2:
-#pragma unroll
-                    for (int k = 0; k < 5; k++) w[k] = x[2 + k];
-                    break;
-                case 3:
-#pragma unroll
-                    for (int k = 0; k < 5; k++) w[k] = x[3 + k];
-                    break;
-                default:  // 4: sh = 16, rb = 0
-#pragma unroll
-                    for (int k = 0; k < 4; k++) w[k] = x[4 + k];
-                    w[4] = 0u;
-                    break;
-                }
-                u32x4 v;
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    v[k] = rb ? __builtin_amdgcn_alignbyte(w[k + 1], w[k], rb) : w[k];
-                const uint64_t gpos = base + 16ull * c;
-                if (gpos >= lo && gpos + 16 <= hi) {
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
-                } else {
-                    for (int i = 0; i < 16; i++)
-                        if (gpos + i >= lo && gpos + i < hi)
-                            buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
-                }
-            }
-        }
-    } else {
-        // the tile's symbols exceed the image region: resolve the offset first, then store
-        // each lane's bytes to global memory
-        if (threadIdx.x < 64) {
-            const uint64_t O = lb_finish(buf.status, tile, tile_syms, wpl, lb);
-            if (threadIdx.x == 0) *s_base = O;
-        }
-        __syncthreads();
-        const uint64_t O = *s_base;
-        if (cnt) {
-            DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
-            dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
-        }
-    }
-    // The next tile's first LDS write (its walk 2) follows its scan barrier, so this tile's
-    // image reads are complete by then; no barrier is needed here.
-}
-
 // LDS bytes of the decoder's image region, from the device's LDS budget: the region that lets
 // 32 / (THREADS / 64) workgroups share a CU next to the table (or fewer when the table is
 // large), capped at what the largest possible tile needs (16 + THREADS * max_symbols + 32,
@@ -988,6 +885,55 @@ __global__ __launch_bounds__(kOffsetThreads) void dec_offsets_kernel(
     }
 }
 
+// One tile of the write kernels: the lane's segment `seg` (staged in `st`) has `cnt` codewords;
+// the tile's output starts at buf.out + O.  Ends with no barrier: the caller's next LDS image
+// write must follow a barrier that every wave reaches after this tile's copy-out.
+template <int THREADS, bool SINGLE>
+__device__ __forceinline__ void dec_write_tile(const DecBuffer &buf, DecTable t,
+                                               const uint32_t *s_tab, uint8_t *s_img,
+                                               uint32_t image_bytes, uint32_t *s_wave,
+                                               const DecStage &st, uint32_t seg, uint32_t cnt,
+                                               uint64_t O) {
+    const uint32_t P = t.prefix_bits;
+    const DecSeg g = dec_segment(buf, st, seg);
+    uint32_t tile_syms;
+    const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
+    const uint32_t lead = (uint32_t)(O & 15u);
+    if (lead + tile_syms + 17u <= image_bytes) {  // uniform
+        // walk 2 into the image; after a barrier every lane writes its first symbol again
+        // over the stray byte its predecessor left there
+        const uint32_t a = lead + excl;
+        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
+        __syncthreads();
+        if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+        __syncthreads();
+        // output bytes [O, O + tile_syms) capped at n: 16-byte chunks of the image, byte
+        // stores in the two chunks shared with neighbouring tiles
+        const uint64_t base = O & ~15ull;
+        uint64_t hi = O + tile_syms;
+        if (hi > buf.n) hi = buf.n;
+        const uint32_t chunks = (lead + tile_syms + 15u) >> 4;
+        for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+            const u32x4 v = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
+            const uint64_t gpos = base + 16ull * c;
+            if (gpos >= O && gpos + 16 <= hi) {
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
+            } else {
+                for (int i = 0; i < 16; i++)
+                    if (gpos + i >= O && gpos + i < hi)
+                        buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+            }
+        }
+    } else {
+        // more symbols than the image holds: each lane stores its bytes to global memory
+        if (cnt) {
+            DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
+            dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
+        }
+        __syncthreads();  // every wave has read this tile's s_wave
+    }
+}
+
 template <int THREADS, bool SINGLE>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void dec_write_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ table,
@@ -998,7 +944,6 @@ void dec_write_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
     __shared__ uint32_t s_wave[THREADS / 64];
     uint32_t *s_tab = s_dyn;
     uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);  // 16-byte aligned
-    const uint32_t P = t.prefix_bits;
     for (uint32_t i = threadIdx.x; i < table_words; i += THREADS) s_tab[i] = table[i];
     __syncthreads();
     uint32_t tile = blockIdx.x;
@@ -1008,44 +953,8 @@ void dec_write_kernel(DecBuffer buf, DecTable t, const uint32_t *__restrict__ ta
         const uint32_t nt = tile + gridDim.x;
         const DecStage sn = dec_fetch<THREADS>(buf, nt);
         const uint32_t cn = nt < buf.num_tiles ? seg_counts[nt * THREADS + threadIdx.x] : 0u;
-        const uint64_t O = tile_offsets[tile];
-        const DecSeg g = dec_segment(buf, st, tile * THREADS + threadIdx.x);
-        uint32_t tile_syms;
-        const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
-        const uint32_t lead = (uint32_t)(O & 15u);
-        if (lead + tile_syms + 17u <= image_bytes) {  // uniform
-            // walk 2 into the image; after a barrier every lane writes its first symbol again
-            // over the stray byte its predecessor left there
-            const uint32_t a = lead + excl;
-            if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
-            __syncthreads();
-            if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
-            __syncthreads();
-            // output bytes [O, O + tile_syms) capped at n: 16-byte chunks of the image, byte
-            // stores in the two chunks shared with neighbouring tiles
-            const uint64_t base = O & ~15ull;
-            uint64_t hi = O + tile_syms;
-            if (hi > buf.n) hi = buf.n;
-            const uint32_t chunks = (lead + tile_syms + 15u) >> 4;
-            for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
-                const u32x4 v = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
-                const uint64_t gpos = base + 16ull * c;
-                if (gpos >= O && gpos + 16 <= hi) {
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
-                } else {
-                    for (int i = 0; i < 16; i++)
-                        if (gpos + i >= O && gpos + i < hi)
-                            buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
-                }
-            }
-        } else {
-            // more symbols than the image holds: each lane stores its bytes to global memory
-            if (cnt) {
-                DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
-                dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
-            }
-            __syncthreads();  // every wave has read this tile's s_wave
-        }
+        dec_write_tile<THREADS, SINGLE>(buf, t, s_tab, s_img, image_bytes, s_wave, st,
+                                        tile * THREADS + threadIdx.x, cnt, tile_offsets[tile]);
         st = sn;
         cnt = cn;
         tile = nt;
@@ -1219,51 +1128,110 @@ struct DecJobDev {
     uint32_t reserved;
 };
 
+// Batched decode: the three single-buffer launches over the tiles of all jobs.  Workgroup b
+// takes the contiguous tiles [b*C, (b+1)*C) (C = ceil(total / grid)), so consecutive tiles of a
+// workgroup mostly share a job and its table is loaded once per job.  The offsets kernel scans
+// all tiles; a tile's offset inside its job is its prefix minus that of the job's first tile.
+struct DecCursor {
+    const DecJobDev *jobs;
+    uint32_t count, j;
+    DecJobDev job;
+    // the job owning tile g (g at or after the current job's first tile)
+    __device__ __forceinline__ void seek(uint32_t g) {
+        while (g >= job.base + job.num_tiles && j + 1 < count) job = jobs[++j];
+    }
+};
+
+__device__ __forceinline__ DecBuffer dec_job_buffer(const DecJobDev &job) {
+    DecBuffer buf;
+    buf.payload = job.payload;
+    buf.payload_words = job.payload_words;
+    buf.gap_elements = job.gap_elements;
+    buf.gap = job.gap;
+    buf.out = job.out;
+    buf.n = job.n;
+    buf.status = nullptr;
+    buf.num_tiles = job.num_tiles;
+    buf.reserved = 0;
+    return buf;
+}
+
+// loads job `j`'s table into LDS (after a barrier: the previous table may still be in use)
+template <int THREADS>
+__device__ __forceinline__ void dec_load_job_table(uint32_t *s_tab, const DecJobDev &job) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < job.table_words; i += THREADS) s_tab[i] = job.table[i];
+    __syncthreads();
+}
+
 template <int THREADS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
-void dec_batch_kernel(
-    const DecJobDev *__restrict__ jobs, uint32_t count, uint32_t total_tiles,
-    uint32_t image_offset, uint32_t image_bytes, uint32_t *__restrict__ counter,
-    uint64_t *__restrict__ status, uint32_t wpl) {
+void dec_count_batch_kernel(const DecJobDev *__restrict__ jobs, uint32_t count, uint32_t total,
+                            uint8_t *__restrict__ seg_counts, uint32_t *__restrict__ wave_sums) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    uint32_t *s_tab = s_dyn;
+    const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t g0 = blockIdx.x * chunk;
+    const uint32_t g1 = g0 + chunk < total ? g0 + chunk : total;
+    if (g0 >= g1) return;  // uniform
+    DecCursor cur{jobs, count, find_job(jobs, count, g0), {}};
+    cur.job = jobs[cur.j];
+    uint32_t loaded = 0xFFFFFFFFu;
+    for (uint32_t g = g0; g < g1; g++) {
+        cur.seek(g);
+        if (cur.j != loaded) {  // uniform
+            dec_load_job_table<THREADS>(s_tab, cur.job);
+            loaded = cur.j;
+        }
+        const DecBuffer buf = dec_job_buffer(cur.job);
+        const uint32_t tile = g - cur.job.base;
+        const DecStage st = dec_fetch<THREADS>(buf, tile);
+        const uint32_t seg = tile * THREADS + threadIdx.x;
+        const DecSeg sg = dec_segment(buf, st, seg);
+        const uint8_t *s_len =
+            reinterpret_cast<const uint8_t *>(s_tab + cur.job.table_words - kDecLengthWords);
+        const uint32_t cnt =
+            seg < buf.gap_elements
+                ? dec_count<false, EntStored>(sg, s_tab, s_len, cur.job.prefix_bits,
+                                              cur.job.l1_entries)
+                : 0u;
+        seg_counts[(uint64_t)g * THREADS + threadIdx.x] = (uint8_t)cnt;
+        const uint32_t wsum = wave_incl_sum(cnt);
+        if (lane_id() == 63) wave_sums[(uint64_t)g * (THREADS / 64) + (threadIdx.x >> 6)] = wsum;
+    }
+}
+
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+void dec_write_batch_kernel(const DecJobDev *__restrict__ jobs, uint32_t count, uint32_t total,
+                            uint32_t image_offset, uint32_t image_bytes,
+                            const uint8_t *__restrict__ seg_counts,
+                            const uint64_t *__restrict__ tile_offsets) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_wave[THREADS / 64];
-    __shared__ uint64_t s_base;
-    __shared__ uint32_t s_tile, s_job;
     uint32_t *s_tab = s_dyn;
     uint8_t *s_img = reinterpret_cast<uint8_t *>(s_dyn + image_offset);
+    const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t g0 = blockIdx.x * chunk;
+    const uint32_t g1 = g0 + chunk < total ? g0 + chunk : total;
+    if (g0 >= g1) return;  // uniform
+    DecCursor cur{jobs, count, find_job(jobs, count, g0), {}};
+    cur.job = jobs[cur.j];
     uint32_t loaded = 0xFFFFFFFFu;
-    while (true) {
-        if (threadIdx.x == 0) {
-            const uint32_t g = atomicAdd(counter, 1u);
-            s_tile = g;
-            if (g < total_tiles) s_job = find_job(jobs, count, g);
+    for (uint32_t g = g0; g < g1; g++) {
+        cur.seek(g);
+        if (cur.j != loaded) {  // uniform; its barriers also end the previous tile's copy-out
+            dec_load_job_table<THREADS>(s_tab, cur.job);
+            loaded = cur.j;
         }
-        __syncthreads();
-        const uint32_t g = s_tile;
-        if (g >= total_tiles) break;
-        const uint32_t j = s_job;
-        const DecJobDev job = jobs[j];
-        if (j != loaded) {  // uniform
-            for (uint32_t i = threadIdx.x; i < job.table_words; i += THREADS) s_tab[i] = job.table[i];
-            loaded = j;
-            __syncthreads();
-        }
-        DecBuffer buf;
-        buf.payload = job.payload;
-        buf.payload_words = job.payload_words;
-        buf.gap_elements = job.gap_elements;
-        buf.gap = job.gap;
-        buf.out = job.out;
-        buf.n = job.n;
-        buf.status = status + job.base;
-        buf.num_tiles = job.num_tiles;
-        buf.reserved = 0;
-        const DecTable t{job.prefix_bits, job.l1_entries};
-        const DecStage st = dec_fetch<THREADS>(buf, g - job.base);
-        const uint8_t *s_len =
-            reinterpret_cast<const uint8_t *>(s_tab + job.table_words - kDecLengthWords);
-        dec_tile<THREADS, false>(buf, g - job.base, st, t, s_tab, s_len, s_img, image_bytes,
-                                 s_wave, &s_base, wpl);
+        const DecBuffer buf = dec_job_buffer(cur.job);
+        const uint32_t tile = g - cur.job.base;
+        const DecStage st = dec_fetch<THREADS>(buf, tile);
+        const uint32_t cnt = seg_counts[(uint64_t)g * THREADS + threadIdx.x];
+        const uint64_t O = tile_offsets[g] - tile_offsets[cur.job.base];
+        const DecTable t{cur.job.prefix_bits, cur.job.l1_entries};
+        dec_write_tile<THREADS, false>(buf, t, s_tab, s_img, image_bytes, s_wave, st,
+                                       tile * THREADS + threadIdx.x, cnt, O);
     }
 }
 
@@ -1455,7 +1423,7 @@ template <int THREADS>
 static uint32_t dec_shape_image(uint32_t table_words, uint32_t max_symbols) {
     size_t st = static_lds(dec_write_kernel<THREADS, true>);
     st = std::max(st, static_lds(dec_write_kernel<THREADS, false>));
-    st = std::max(st, static_lds(dec_batch_kernel<THREADS>));
+    st = std::max(st, static_lds(dec_write_batch_kernel<THREADS>));
     return dec_image_bytes(THREADS, table_words, max_symbols, st);
 }
 
@@ -1598,20 +1566,52 @@ static hgap_status launch_enc_batch(const EncJobDev *jobs, uint32_t count, uint3
     return HGAP_ERR_UNSUPPORTED;
 }
 
+// Workspace of a batched decode: the offsets kernel's ticket and status words, one count byte
+// per segment, one sum per 64 segments, one 64-bit prefix per tile (of the whole batch).
+struct DecBatchWork {
+    DecWork w;
+    uint64_t bytes;
+};
+
+static DecBatchWork dec_batch_work(void *ws, uint64_t total_tiles, uint32_t threads) {
+    const uint64_t blocks = (total_tiles + kOffsetTiles - 1) / kOffsetTiles;
+    const uint64_t status = (8 * blocks + 15) & ~15ull;
+    const uint64_t counts = (total_tiles * threads + 15) & ~15ull;
+    const uint64_t sums = (total_tiles * (threads / 64) * 4 + 15) & ~15ull;
+    uint8_t *p = static_cast<uint8_t *>(ws);
+    DecBatchWork r;
+    r.w.counter = reinterpret_cast<uint32_t *>(p);
+    r.w.status = reinterpret_cast<uint64_t *>(p + 16);
+    r.w.seg_counts = p + 16 + status;
+    r.w.wave_sums = reinterpret_cast<uint32_t *>(p + 16 + status + counts);
+    r.w.tile_offsets = reinterpret_cast<uint64_t *>(p + 16 + status + counts + sums);
+    r.bytes = 16 + status + counts + sums + 8 * total_tiles;
+    return r;
+}
+
 template <int THREADS>
 static hgap_status launch_dec_batch_threads(const DecJobDev *jobs, uint32_t count,
                                             uint32_t total, uint32_t max_table_words,
-                                            uint32_t max_symbols, uint32_t *ws,
+                                            uint32_t max_symbols, const DecWork &w,
                                             const hgap_launch_config &c, hipStream_t s) {
     const uint32_t image = dec_shape_image<THREADS>(max_table_words, max_symbols);
     if (image == 0) return HGAP_ERR_UNSUPPORTED;
     const uint32_t image_offset = dec_image_offset(max_table_words);
-    const size_t lds = 4ull * image_offset + image;
-    auto kern = dec_batch_kernel<THREADS>;
-    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
-    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total, image_offset,
-                       image, ws, reinterpret_cast<uint64_t *>(ws + 4), window_per_lane(c));
+    auto count_k = dec_count_batch_kernel<THREADS>;
+    auto write_k = dec_write_batch_kernel<THREADS>;
+    const size_t lds_count = 4ull * max_table_words;
+    const size_t lds_write = 4ull * image_offset + image;
+    if (!allow_dynamic_lds(count_k, lds_count) || !allow_dynamic_lds(write_k, lds_write))
+        return HGAP_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL(count_k, dim3(grid_for(count_k, THREADS, lds_count, c.blocks_per_cu, total)),
+                       dim3(THREADS), lds_count, s, jobs, count, total, w.seg_counts, w.wave_sums);
+    const unsigned blocks = (total + kOffsetTiles - 1) / kOffsetTiles;
+    hipLaunchKernelGGL(dec_offsets_kernel<THREADS>, dim3(blocks), dim3(kOffsetThreads), 0, s,
+                       w.wave_sums, total, w.tile_offsets, w.status, w.counter,
+                       window_per_lane(c));
+    hipLaunchKernelGGL(write_k, dim3(grid_for(write_k, THREADS, lds_write, c.blocks_per_cu, total)),
+                       dim3(THREADS), lds_write, s, jobs, count, total, image_offset, image,
+                       w.seg_counts, w.tile_offsets);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
@@ -1889,6 +1889,12 @@ uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
 
 size_t hgap_batch_workspace_bytes(uint32_t total_tiles) { return 16 + 8ull * total_tiles; }
 
+size_t hgap_batch_decode_workspace_bytes(uint32_t total_tiles, const hgap_launch_config *cfg) {
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return 0;
+    return dec_batch_work(nullptr, total_tiles, c.dec_threads).bytes;
+}
+
 hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32_t total_tiles,
                               uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
                               size_t workspace_bytes, const hgap_launch_config *cfg,
@@ -1922,22 +1928,24 @@ hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32
     if (max_symbols < 1 || max_symbols > HGAP_SEGMENT_BITS ||
         max_table_words < 2 + kDecLengthWords)
         return HGAP_ERR_CODEBOOK;
-    if (workspace_bytes < hgap_batch_workspace_bytes(total_tiles)) return HGAP_ERR_WORKSPACE;
+    const DecBatchWork bw = dec_batch_work(d_workspace, total_tiles, c.dec_threads);
+    if (workspace_bytes < bw.bytes) return HGAP_ERR_WORKSPACE;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
     const DecJobDev *jobs = reinterpret_cast<const DecJobDev *>(d_jobs);
-    hgap_status st = launch_zero(ws, 4 + 2ull * total_tiles, nullptr, 0, s);
+    // the offsets kernel's ticket and look-back status start at zero
+    const uint64_t blocks = ((uint64_t)total_tiles + kOffsetTiles - 1) / kOffsetTiles;
+    hgap_status st = launch_zero(bw.w.counter, 4 + 2 * blocks, nullptr, 0, s);
     if (st != HGAP_OK) return st;
     switch (c.dec_threads) {
     case 256:
         return launch_dec_batch_threads<256>(jobs, count, total_tiles, max_table_words,
-                                             max_symbols, ws, c, s);
+                                             max_symbols, bw.w, c, s);
     case 512:
         return launch_dec_batch_threads<512>(jobs, count, total_tiles, max_table_words,
-                                             max_symbols, ws, c, s);
+                                             max_symbols, bw.w, c, s);
     case 1024:
         return launch_dec_batch_threads<1024>(jobs, count, total_tiles, max_table_words,
-                                              max_symbols, ws, c, s);
+                                              max_symbols, bw.w, c, s);
     default: return HGAP_ERR_UNSUPPORTED;
     }
 }

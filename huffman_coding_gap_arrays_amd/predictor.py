@@ -51,7 +51,7 @@ N_ENC, N_DEC = len(ENC_CANDIDATES), len(DEC_CANDIDATES)
 N_OUT = N_ENC + N_DEC
 
 DEFAULT_ENC = dict(enc_threads=512, enc_bytes_per_thread=64, lookback_window=64)
-DEFAULT_DEC = dict(dec_threads=512, lookback_window=64, prefix_bits=11)
+DEFAULT_DEC = dict(dec_threads=256, lookback_window=64, prefix_bits=11)
 
 
 def candidate_index(cfg, candidates):

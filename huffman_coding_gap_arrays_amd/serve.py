@@ -87,7 +87,11 @@ def encode_reserve_exact(n, sizes):
 
 
 def decode_reserve(n, gap_elements):
-    return _up(ALIGN, n + 16) + 8 * (gap_elements // 256 + 2) + 4 * ALIGN
+    """Output, plus the request's share of the batched decoder's workspace: one count byte and
+    1/16 byte of wave sums per segment (whole 1024-segment tiles), one offset per 256-segment
+    tile, the offsets scan's status words."""
+    segs = (gap_elements + 1023) // 1024 * 1024
+    return _up(ALIGN, n + 16) + segs + segs // 16 + 8 * (gap_elements // 256 + 2) + 64 + 4 * ALIGN
 
 
 def request_bytes(r):
@@ -188,8 +192,8 @@ class _Batch:
                 max_sym = max(max_sym, info.max_symbols_per_segment)
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(members), ctypes.byref(cfg))
             host, d_jobs = _pinned_to_device(jobs, s.device)
-            ws = torch.empty(_up(16, lib.hgap_batch_workspace_bytes(total)), dtype=torch.uint8,
-                             device=s.device)
+            ws = torch.empty(_up(16, lib.hgap_batch_decode_workspace_bytes(total, ctypes.byref(cfg))),
+                             dtype=torch.uint8, device=s.device)
             self.keep += [host, d_jobs, ws, cfg]
             check(lib.hgap_batch_decode(ctypes.c_void_p(d_jobs.data_ptr()), len(members), total,
                                         max_tw, max_sym, ctypes.c_void_p(ws.data_ptr()),

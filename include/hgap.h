@@ -236,7 +236,12 @@ uint32_t hgap_batch_plan_encode(hgap_enc_job *jobs, uint32_t count,
                                 const hgap_launch_config *cfg);
 uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
                                 const hgap_launch_config *cfg);
+/* Workspace of hgap_batch_encode (look-back status words of `total_tiles` tiles). */
 size_t hgap_batch_workspace_bytes(uint32_t total_tiles);
+/* Workspace of hgap_batch_decode for `total_tiles` tiles of the shape in `cfg` (NULL =
+ * default): per-segment codeword counts, per-tile offsets and the offsets scan's status words
+ * (about 1.1 bytes per 128-bit segment).  0 for an invalid shape. */
+size_t hgap_batch_decode_workspace_bytes(uint32_t total_tiles, const hgap_launch_config *cfg);
 /* max_len / max_table_words / max_symbols: maxima over the jobs (they size LDS). */
 hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32_t total_tiles,
                               uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
