@@ -110,6 +110,14 @@ def _config_arrays(configs, B):
     return out
 
 
+def expected_symbols(n, gap_elements, max_symbols):
+    """Codewords per segment the batched decoder sizes its LDS image for: 1.25 x the stream's
+    mean + 2, at most the table's worst case (denser tiles take a slower path)."""
+    n = np.asarray(n, np.int64)
+    g = np.maximum(np.asarray(gap_elements, np.int64), 1)
+    return np.minimum(np.asarray(max_symbols, np.int64), (5 * n + 4 * g - 1) // (4 * g) + 2)
+
+
 class BatchCodec:
     """Encodes and decodes a list of device uint8 buffers in batched launches."""
 
@@ -231,12 +239,13 @@ class BatchCodec:
             jobs["l1_entries"] = iv["l1_entries"][idx]
             jobs["table_words"] = tw[idx]
             jobs["max_symbols"] = ms[idx]
+            dense = expected_symbols(self.ns[idx], self.gap_elements[idx], ms[idx])
             jobs["max_length"] = iv["max_length"][idx]
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(idx), ctypes.byref(cfg))
             self.dec_launches.append(dict(cfg=cfg, jobs=_to_device(jobs, self.device),
                                           count=len(idx), total=total,
                                           max_tw=int(max(2, tw[idx].max())),
-                                          max_sym=int(max(1, ms[idx].max())),
+                                          max_sym=int(max(1, dense.max())),
                                           ws=self._workspace(total, cfg), members=idx))
 
     def encode(self, stream=None):

@@ -1400,11 +1400,12 @@ static size_t static_lds(K kernel) {
     return attr.sharedSizeBytes;
 }
 
-// Image region of a decoder launch: the most that lets the most workgroups of `threads` lanes
-// share a CU (32 waves per CU, LDS blocks of 512 bytes) next to the table and the kernel's
-// static LDS, capped at what the largest possible tile needs; at least min(need, 16 * threads
-// + 48) bytes, else 0 (the shape does not fit).  Tiles whose symbols exceed the region take the
-// decoder's global-store path.
+// Image region of a decoder launch for segments of up to `max_symbols` codewords: the region
+// of the most workgroups of `threads` lanes per CU (32 waves per CU, LDS blocks of 512 bytes)
+// that holds such a tile next to the table and the kernel's static LDS (capped at that need),
+// else the largest region one workgroup can have; 0 when not even min(need, 16 * threads + 48)
+// bytes fit (the shape does not fit).  Tiles whose symbols exceed the region take the decoder's
+// global-store path.
 static uint32_t dec_image_bytes(uint32_t threads, uint32_t table_words, uint32_t max_symbols,
                                 size_t static_bytes) {
     const uint32_t need = dec_image_need(threads, max_symbols);
@@ -1412,7 +1413,8 @@ static uint32_t dec_image_bytes(uint32_t threads, uint32_t table_words, uint32_t
     const size_t fixed = static_bytes + 4ull * dec_image_offset(table_words);
     for (uint32_t wgs = 32u / (threads / 64u); wgs >= 1; wgs--) {
         const size_t per_wg = (kMaxLds / wgs) & ~(size_t)511;
-        if (per_wg < fixed + minimum) continue;
+        if (per_wg < fixed + need && wgs > 1) continue;
+        if (per_wg < fixed + minimum) return 0;
         const size_t budget = (per_wg - fixed) & ~(size_t)15;
         return (uint32_t)std::min<size_t>(need, budget);
     }
@@ -1803,7 +1805,12 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     b.status = nullptr;
     b.num_tiles = (uint32_t)tiles;
     b.reserved = 0;
-    return launch_dec(b, static_cast<const uint32_t *>(d_table), *info, w, c, s);
+    // size the LDS image for this stream's density (1.25 x its mean codewords per segment, + 2)
+    // rather than the table's worst case: denser tiles take the global-store path
+    hgap_dectable_info sized = *info;
+    const uint64_t expect = (5ull * n + 4ull * gap_elements - 1) / (4ull * gap_elements) + 2;
+    if (expect < sized.max_symbols_per_segment) sized.max_symbols_per_segment = (uint32_t)expect;
+    return launch_dec(b, static_cast<const uint32_t *>(d_table), sized, w, c, s);
 }
 
 // Default shape first; when table + staging do not fit in LDS, fewer segments per tile.

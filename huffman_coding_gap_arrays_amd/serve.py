@@ -189,7 +189,8 @@ class _Batch:
                 jobs[j]["max_symbols"] = info.max_symbols_per_segment
                 jobs[j]["max_length"] = info.max_length
                 max_tw = max(max_tw, info.bytes // 4)
-                max_sym = max(max_sym, info.max_symbols_per_segment)
+                max_sym = max(max_sym, int(batch.expected_symbols(
+                    r.n, e.gap_elements, info.max_symbols_per_segment)))
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(members), ctypes.byref(cfg))
             host, d_jobs = _pinned_to_device(jobs, s.device)
             ws = torch.empty(_up(16, lib.hgap_batch_decode_workspace_bytes(total, ctypes.byref(cfg))),

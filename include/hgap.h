@@ -242,7 +242,9 @@ size_t hgap_batch_workspace_bytes(uint32_t total_tiles);
  * default): per-segment codeword counts, per-tile offsets and the offsets scan's status words
  * (about 1.1 bytes per 128-bit segment).  0 for an invalid shape. */
 size_t hgap_batch_decode_workspace_bytes(uint32_t total_tiles, const hgap_launch_config *cfg);
-/* max_len / max_table_words / max_symbols: maxima over the jobs (they size LDS). */
+/* max_len / max_table_words: maxima over the jobs (they size LDS).  max_symbols: the codewords
+ * per segment the decoder's LDS image is sized for (the jobs' max_symbols_per_segment, or less:
+ * tiles with more codewords than fit take a slower path, with the same output). */
 hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32_t total_tiles,
                               uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
                               size_t workspace_bytes, const hgap_launch_config *cfg,
