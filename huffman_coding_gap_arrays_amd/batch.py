@@ -222,7 +222,7 @@ class BatchCodec:
         ms = iv["max_symbols_per_segment"].astype(np.int64)
         tw = iv["bytes"].astype(np.int64) // 4  # words of each table image
         # one launch per decoder shape; its LDS image is sized from the members' largest table
-        # and densest segments (tiles that outgrow it take the kernel's global-store path)
+        # and densest segments (tiles that outgrow it go through the region in windows)
         dkeys = np.stack([d["dec_threads"], d["lookback_window"]], 1)
         for key in np.unique(dkeys[live], axis=0):
             idx = live[(dkeys[live] == key).all(1)]

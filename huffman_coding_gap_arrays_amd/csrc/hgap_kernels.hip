@@ -576,10 +576,8 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
 //   4. the look-back wave resolves the tile's output offset O and claims the next tile;
 //   5. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
 //      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
-// The image region holds `image_bytes` bytes (chosen by the launcher for occupancy).  A tile
-// whose symbols do not fit skips the image: after its look-back, walk 2 stores its bytes to
-// global memory directly (slow, but only for tiles averaging far more symbols per segment than
-// the region was sized for).
+// The image region holds `image_bytes` bytes (chosen by the launcher for occupancy); a tile
+// whose symbols do not fit goes through it in windows.
 struct DecBuffer {
     const uint32_t *payload;
     uint32_t payload_words;
@@ -765,20 +763,17 @@ struct DecLdsOut {
     __device__ __forceinline__ void advance(uint32_t d) { p += d; }
 };
 
-// Global-memory writer for walk 2 of a tile that skips the LDS image: stores symbol i of the
-// lane at out[o + i] when i < cnt and o + i < n (the stray byte is never stored).
-struct DecGlobalOut {
-    uint8_t *out;
-    uint64_t o, n;
-    uint32_t i, cnt;
-    __device__ __forceinline__ void put(uint32_t k, uint8_t v) {
-        if (i + k < cnt && o + i + k < n) out[o + i + k] = v;
-    }
+// LDS writer for walk 2 of a tile written through windows of the image region: only bytes
+// whose position x (relative to the window's first image byte) lies in [0, span) are stored.
+struct DecLdsWindow {
+    uint8_t *img;
+    int32_t x;
+    uint32_t span;
     __device__ __forceinline__ void put2(uint32_t ent) {
-        put(0, (uint8_t)ent);
-        put(1, (uint8_t)(ent >> 16));
+        if ((uint32_t)x < span) img[x] = (uint8_t)ent;
+        if ((uint32_t)(x + 1) < span) img[x + 1] = (uint8_t)(ent >> 16);
     }
-    __device__ __forceinline__ void advance(uint32_t d) { i += d; }
+    __device__ __forceinline__ void advance(uint32_t d) { x += (int32_t)d; }
 };
 
 // LDS bytes of the decoder's image region, from the device's LDS budget: the region that lets
@@ -797,7 +792,8 @@ __host__ __device__ constexpr uint32_t dec_image_need(uint32_t threads, uint32_t
 //                       1024 tiles chained by the look-back)
 //   dec_write_kernel    per tile: scan of its segment counts, walk 2 into an LDS image aligned
 //                       like the output (image byte j = output byte (O & ~15) + j), aligned
-//                       128-bit stores
+//                       128-bit stores (a tile larger than the image region goes through it in
+//                       windows)
 // The payload is read twice; in exchange no tile waits on another, and the persistent
 // workgroups take tiles blockIdx.x, blockIdx.x + gridDim.x, ... without a ticket, loading the
 // next tile's inputs into registers one tile ahead.
@@ -885,6 +881,32 @@ __global__ __launch_bounds__(kOffsetThreads) void dec_offsets_kernel(
     }
 }
 
+// Stores image bytes [j0, j1) of a tile (j0 a multiple of 16; the image region holds them from
+// its start) to output bytes base + [j0, j1), base = O & ~15, keeping only [O, O + tile_syms)
+// below n: 16-byte stores, byte stores in chunks that are not wholly inside.
+template <int THREADS>
+__device__ __forceinline__ void dec_copy_out(const DecBuffer &buf, const uint8_t *s_img,
+                                             uint64_t O, uint32_t tile_syms, uint32_t j0,
+                                             uint32_t j1) {
+    const uint64_t base = O & ~15ull;
+    uint64_t lo = base + j0 > O ? base + j0 : O;
+    uint64_t hi = O + tile_syms;
+    if (hi > base + j1) hi = base + j1;
+    if (hi > buf.n) hi = buf.n;
+    const uint32_t chunks = (j1 - j0 + 15u) >> 4;
+    for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
+        const uint64_t gpos = base + j0 + 16ull * c;
+        if (gpos >= lo && gpos + 16 <= hi) {
+            __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
+        } else {
+            for (int i = 0; i < 16; i++)
+                if (gpos + i >= lo && gpos + i < hi)
+                    buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
+        }
+    }
+}
+
 // One tile of the write kernels: the lane's segment `seg` (staged in `st`) has `cnt` codewords;
 // the tile's output starts at buf.out + O.  Ends with no barrier: the caller's next LDS image
 // write must follow a barrier that every wave reaches after this tile's copy-out.
@@ -899,38 +921,31 @@ __device__ __forceinline__ void dec_write_tile(const DecBuffer &buf, DecTable t,
     uint32_t tile_syms;
     const uint32_t excl = block_excl_sum<THREADS>(cnt, s_wave, tile_syms);
     const uint32_t lead = (uint32_t)(O & 15u);
-    if (lead + tile_syms + 17u <= image_bytes) {  // uniform
+    const uint32_t a = lead + excl;  // this lane's first image byte
+    const uint32_t total = lead + tile_syms;
+    if (total + 17u <= image_bytes) {  // uniform
         // walk 2 into the image; after a barrier every lane writes its first symbol again
         // over the stray byte its predecessor left there
-        const uint32_t a = lead + excl;
         if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
         __syncthreads();
         if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
         __syncthreads();
-        // output bytes [O, O + tile_syms) capped at n: 16-byte chunks of the image, byte
-        // stores in the two chunks shared with neighbouring tiles
-        const uint64_t base = O & ~15ull;
-        uint64_t hi = O + tile_syms;
-        if (hi > buf.n) hi = buf.n;
-        const uint32_t chunks = (lead + tile_syms + 15u) >> 4;
-        for (uint32_t c = threadIdx.x; c < chunks; c += THREADS) {
-            const u32x4 v = *reinterpret_cast<const u32x4 *>(s_img + 16 * c);
-            const uint64_t gpos = base + 16ull * c;
-            if (gpos >= O && gpos + 16 <= hi) {
-                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(buf.out + gpos));
-            } else {
-                for (int i = 0; i < 16; i++)
-                    if (gpos + i >= O && gpos + i < hi)
-                        buf.out[gpos + i] = (uint8_t)(v[i >> 2] >> (8 * (i & 3)));
-            }
-        }
+        dec_copy_out<THREADS>(buf, s_img, O, tile_syms, 0u, total);
     } else {
-        // more symbols than the image holds: each lane stores its bytes to global memory
-        if (cnt) {
-            DecGlobalOut w{buf.out, O + excl, buf.n, 0u, cnt};
-            dec_write<SINGLE>(g, s_tab, P, t.l1_entries, w);
+        // more symbols than the region holds: windows of `span` image bytes, each written by
+        // the lanes whose bytes meet it (bytes outside the window are not stored), then stored
+        const uint32_t span = image_bytes & ~15u;
+        for (uint32_t j0 = 0; j0 < total; j0 += span) {  // uniform
+            if (cnt && a < j0 + span && a + cnt > j0)
+                dec_write<SINGLE>(g, s_tab, P, t.l1_entries,
+                                  DecLdsWindow{s_img, (int32_t)(a - j0), span});
+            __syncthreads();
+            if (cnt && a >= j0 && a < j0 + span)
+                s_img[a - j0] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+            __syncthreads();
+            dec_copy_out<THREADS>(buf, s_img, O, tile_syms, j0, std::min(total, j0 + span));
+            __syncthreads();  // the next window overwrites the region
         }
-        __syncthreads();  // every wave has read this tile's s_wave
     }
 }
 
@@ -1404,8 +1419,8 @@ static size_t static_lds(K kernel) {
 // of the most workgroups of `threads` lanes per CU (32 waves per CU, LDS blocks of 512 bytes)
 // that holds such a tile next to the table and the kernel's static LDS (capped at that need),
 // else the largest region one workgroup can have; 0 when not even min(need, 16 * threads + 48)
-// bytes fit (the shape does not fit).  Tiles whose symbols exceed the region take the decoder's
-// global-store path.
+// bytes fit (the shape does not fit).  Tiles whose symbols exceed the region go through it in
+// windows (one more walk per window for the lanes that meet it).
 static uint32_t dec_image_bytes(uint32_t threads, uint32_t table_words, uint32_t max_symbols,
                                 size_t static_bytes) {
     const uint32_t need = dec_image_need(threads, max_symbols);
@@ -1806,7 +1821,7 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     b.num_tiles = (uint32_t)tiles;
     b.reserved = 0;
     // size the LDS image for this stream's density (1.25 x its mean codewords per segment, + 2)
-    // rather than the table's worst case: denser tiles take the global-store path
+    // rather than the table's worst case: denser tiles go through the region in windows
     hgap_dectable_info sized = *info;
     const uint64_t expect = (5ull * n + 4ull * gap_elements - 1) / (4ull * gap_elements) + 2;
     if (expect < sized.max_symbols_per_segment) sized.max_symbols_per_segment = (uint32_t)expect;

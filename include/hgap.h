@@ -262,8 +262,8 @@ hgap_status hgap_config_supported(const hgap_launch_config *cfg, uint32_t max_le
                                   const hgap_dectable_info *info);
 /* LDS bytes of the decoder's per-tile output image for `dec_threads` segments per tile, a
  * table image of `table_words` words and segments of at most `max_symbols` codewords (0: the
- * shape does not fit).  Sized for occupancy; tiles with more symbols than fit take a slower
- * global-store path.  No counterpart in the reference. */
+ * shape does not fit).  Sized for occupancy; tiles with more symbols than fit go through it
+ * in windows (slower, same output).  No counterpart in the reference. */
 uint32_t hgap_decode_image_bytes(uint32_t dec_threads, uint32_t table_words,
                                  uint32_t max_symbols);
 

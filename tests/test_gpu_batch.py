@@ -84,6 +84,21 @@ def test_batched_encode_decode_bit_exact(batch_data, orc, mixed):
     assert bc.mismatches() == 0
 
 
+def test_batched_decode_through_small_image(batch_data, monkeypatch):
+    """A batched decode whose LDS image is sized for one codeword per segment sends every tile
+    through the image in windows; every buffer still decodes to its input."""
+    import torch
+
+    batch, codec, arena, bufs = batch_data
+    monkeypatch.setattr(batch, "expected_symbols",
+                        lambda n, g, m: np.ones(np.asarray(n).shape, np.int64))
+    say("batch codec, one-codeword image")
+    bc = batch.BatchCodec(bufs)
+    bc.run()
+    torch.cuda.synchronize()
+    assert bc.mismatches() == 0
+
+
 def test_batched_graph_replays(batch_data):
     """The captured graph re-runs encode + decode exactly, replay after replay, with eager work
     and synchronizations in between."""

@@ -149,6 +149,30 @@ def test_decode_prefix_splits(codec, orc, prefix):
     assert bool((out == d_dev).all())
 
 
+def test_decode_dense_tiles_through_windows(codec):
+    """Half zero bytes (a 1-bit code: about 128 codewords per segment), half uniform bytes: the
+    decoder sizes its LDS image for the stream's mean density, so the dense tiles go through the
+    image in windows.  The output is the input."""
+    import torch
+
+    n = (3 << 20) + 11
+    half = (n // 2) & ~15
+    d = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    cdf, perm = dist_spec("uniform")
+    codec.synth_fill(d[half:], 5, cdf, perm)
+    enc = codec.encode(d)
+    table = codec.DeviceTable(enc.codebook)
+    info = table.info
+    assert info.min_length == 1
+    mean = n / enc.gap_elements
+    expect = min(info.max_symbols_per_segment, -(-5 * n // (4 * enc.gap_elements)) + 2)
+    image = codec._lib().hgap_decode_image_bytes(256, info.bytes // 4, expect)
+    say("dense tiles: mean", round(mean, 1), "codewords/segment, image", image, "bytes")
+    assert 0 < image < 256 * 100  # a tile of the zero half holds ~256 x 128 codewords
+    out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
+    assert torch.equal(out[:n], d)
+
+
 def test_repeated_calls_reuse_workspace(codec):
     """Look-back status words are cleared on every call: results repeat exactly."""
     import torch
