@@ -110,35 +110,16 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The look-back in two halves, so that its first round trip overlaps other work of the
-// workgroup: lb_start publishes the tile's aggregate (the inclusive prefix for tile 0) and
-// issues the load of each lane's first window word (the whole first window when wpl = 1);
-// lb_finish completes the walk and publishes the inclusive prefix.  Both are called by all 64
-// lanes of one wave.
-struct LookBack {
-    uint64_t v0;
-};
-
-__device__ __forceinline__ void lb_start(uint64_t *status, uint32_t tile, uint64_t aggregate,
-                                         uint32_t wpl, LookBack &lb) {
+__device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate,
+                              uint32_t wpl) {
     const uint32_t lane = lane_id();
     if (tile == 0) {
         if (lane == 0) status_store(&status[0], kFlagP | aggregate);
-        return;
+        return 0;
     }
     if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
-    const int64_t first = (int64_t)tile - 1 - (int64_t)(lane * wpl);
-    // before tile 0: an empty inclusive prefix
-    lb.v0 = first >= 0 ? status_load(&status[first]) : kFlagP;
-}
-
-__device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregate, uint32_t wpl,
-                              LookBack &lb) {
-    if (tile == 0) return 0;
-    const uint32_t lane = lane_id();
     uint64_t excl = 0;
     int64_t end = (int64_t)tile - 1;
-    bool loaded = true;  // the first window was loaded by lb_start
     while (true) {
         const int64_t first = end - (int64_t)(lane * wpl);
         uint64_t v[kMaxWindowPerLane];
@@ -146,11 +127,8 @@ __device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregat
         for (uint32_t j = 0; j < kMaxWindowPerLane; j++) {
             const int64_t idx = first - (int64_t)j;
             // outside the window: an empty aggregate; before tile 0: an empty inclusive prefix
-            v[j] = loaded && j == 0
-                       ? lb.v0
-                       : (j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP));
+            v[j] = j >= wpl ? kFlagA : (idx >= 0 ? status_load(&status[idx]) : kFlagP);
         }
-        loaded = false;
         while (true) {
             bool missing = false;
 #pragma unroll
@@ -182,18 +160,10 @@ __device__ uint64_t lb_finish(uint64_t *status, uint32_t tile, uint64_t aggregat
     return excl;
 }
 
-__device__ __forceinline__ uint64_t look_back(uint64_t *status, uint32_t tile,
-                                              uint64_t aggregate, uint32_t wpl) {
-    LookBack lb;
-    lb_start(status, tile, aggregate, wpl, lb);
-    return lb_finish(status, tile, aggregate, wpl, lb);
-}
-
 // ---------------------------------------------------------------------------------------------
-// zero fill (used for the histogram, the gap array and the look-back workspace)
-// Where a tile claims the workgroup's next tile: after its own look-back, so tiles are claimed
-// in the order they finish looking back (a tile never waits on a predecessor that has not
-// started), and the next tile's loads overlap this tile's compaction and stores.
+// Where an encoder tile claims the workgroup's next tile: after its own look-back, so tiles are
+// claimed in the order they finish looking back (a tile never waits on a predecessor that has
+// not started), and the next tile's input copy overlaps this tile's packing and stores.
 struct NextTile {
     uint32_t *counter;  // ticket counter; nullptr = the caller claims tiles itself
     uint32_t *s_ticket; // LDS word broadcasting the ticket
@@ -201,6 +171,7 @@ struct NextTile {
     uint8_t *s_stage;   // encoder: LDS staging area the next tile's input is copied into
 };
 
+// zero fill (the histogram, the gap array and the look-back workspaces)
 __global__ void fill_zero_kernel(uint32_t *a, uint64_t na, uint32_t *b, uint64_t nb) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += stride) {
@@ -213,14 +184,18 @@ __global__ void fill_zero_kernel(uint32_t *a, uint64_t na, uint32_t *b, uint64_t
 
 // ---------------------------------------------------------------------------------------------
 // Histogram (replaces cu_histgram + mergeHistogram, encoder/src/encoder.cu:33-116).
-// Every lane index owns one 256-bin array in LDS (shared by the 4 waves of the workgroup, so
-// updates are LDS atomics, but lanes of one wave never collide on an address).  Bin b of copy
-// c sits at word c*256 + (b ^ c): equal bytes in different lanes fall in different banks.
+// Every lane index owns one 256-bin array in LDS (shared by the waves of the workgroup, so
+// updates are LDS atomics, but lanes of one wave never collide on an address).  Bin b of lane
+// l's copy sits at word 64 b + l: whatever bytes the lanes count, they hit distinct banks.
+// The single-buffer kernel runs 1024-lane workgroups (two per CU next to 64 KiB of bins) so
+// that enough 16-byte loads are in flight to cover HBM latency.
 constexpr int kHistCopies = 64;
+constexpr int kHistThreadsSingle = 1024;
 
-__global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__restrict__ in,
-                                                            uint64_t n,
-                                                            uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(kHistThreadsSingle) void hist_kernel(const uint8_t *__restrict__ in,
+                                                                  uint64_t n,
+                                                                  uint32_t *__restrict__ hist) {
+    constexpr int kHistThreads = kHistThreadsSingle;
     __shared__ uint32_t s_bins[kHistCopies * 256];
     for (int i = threadIdx.x; i < kHistCopies * 256; i += kHistThreads) s_bins[i] = 0u;
     __syncthreads();
@@ -255,11 +230,14 @@ __global__ __launch_bounds__(kHistThreads) void hist_kernel(const uint8_t *__res
         if (t < n) atomicAdd(&mine[(uint32_t)in[t] << 6], 1u);
     }
     __syncthreads();
-    // thread b owns bin b: sum of the 64 copies (read in a rotated order so that the lanes of a
-    // pass hit different banks), then one global atomic per bin per workgroup
-    uint32_t sum = 0;
-    for (int c = 0; c < kHistCopies; c++) sum += s_bins[(threadIdx.x << 6) + ((c + threadIdx.x) & 63u)];
-    if (sum) atomicAdd(&hist[threadIdx.x], sum);
+    // thread b < 256 owns bin b: sum of the 64 copies (read in a rotated order so that the
+    // lanes of a pass hit different banks), then one global atomic per bin per workgroup
+    if (threadIdx.x < 256) {
+        uint32_t sum = 0;
+        for (int c = 0; c < kHistCopies; c++)
+            sum += s_bins[(threadIdx.x << 6) + ((c + threadIdx.x) & 63u)];
+        if (sum) atomicAdd(&hist[threadIdx.x], sum);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1704,10 +1682,10 @@ hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
     hipStream_t s = static_cast<hipStream_t>(stream);
     hgap_status st = launch_zero(d_hist, 256, nullptr, 0, s);
     if (st != HGAP_OK || n == 0) return st;
-    uint64_t want = ((n >> 4) + kHistThreads * 4 - 1) / (kHistThreads * 4);
+    uint64_t want = ((n >> 4) + kHistThreadsSingle * 4 - 1) / (kHistThreadsSingle * 4);
     if (want < 1) want = 1;
-    const unsigned blocks = grid_for(hist_kernel, kHistThreads, 0, 0, want);
-    hipLaunchKernelGGL(hist_kernel, dim3(blocks), dim3(kHistThreads), 0, s,
+    const unsigned blocks = grid_for(hist_kernel, kHistThreadsSingle, 0, 0, want);
+    hipLaunchKernelGGL(hist_kernel, dim3(blocks), dim3(kHistThreadsSingle), 0, s,
                        static_cast<const uint8_t *>(d_in), n, d_hist);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
