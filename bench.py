@@ -197,7 +197,8 @@ def main():
     codec.synth_fill(d_in, seed, cdf, perm)
 
     # sizes/buffers from one untimed pass
-    hist = codec.host_histogram(codec.histogram(d_in))
+    hist_dev, range_hist = codec.histogram_ranges(d_in)
+    hist = codec.host_histogram(hist_dev)
     cb = codec.codebook_build(hist)
     sizes = codec.encode_sizes(cb, hist)
     W, G, GW = int(sizes.payload_words), int(sizes.gap_elements), int(sizes.gap_words)
@@ -215,7 +216,7 @@ def main():
         e = [ev() for _ in range(5)] if record else None
         if record:
             e[0].record(stream)
-        h = codec.histogram(d_in, stream)
+        h, rh = codec.histogram_ranges(d_in, stream, range_hist=range_hist)
         if record:
             e[1].record(stream)
         hb = codec.host_histogram(h)  # D2H of 256 counts (synchronizes), as the reference does
@@ -225,7 +226,7 @@ def main():
         if record:
             e[2].record(stream)
         enc = codec.encode(d_in, cb=book, sizes=sz, stream=stream, workspace=ws,
-                           payload=payload, gap=gap)
+                           payload=payload, gap=gap, range_hist=rh)
         if record:
             e[3].record(stream)
         codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n,

@@ -196,6 +196,13 @@ PROTOTYPES = {
         [_P, _U64, ctypes.POINTER(Codebook), ctypes.POINTER(Sizes), _P, _P, _P, _SZ,
          ctypes.c_void_p, _P],
     ),
+    "hgap_range_count": (_U32, [_U64]),
+    "hgap_histogram_ranges": (_INT, [_P, _U64, _P, _P, _P]),
+    "hgap_encode_ranges": (
+        _INT,
+        [_P, _U64, ctypes.POINTER(Codebook), ctypes.POINTER(Sizes), _P, _P, _P, _P, _SZ,
+         ctypes.c_void_p, _P],
+    ),
     "hgap_decode_workspace_bytes": (_SZ, [_U32]),
     "hgap_decode_cfg": (
         _INT,

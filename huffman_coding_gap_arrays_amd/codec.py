@@ -94,6 +94,20 @@ def histogram(d_in, stream=None):
     return hist
 
 
+def histogram_ranges(d_in, stream=None, range_hist=None):
+    """(hist, range_hist): the 256 byte counts, and the counts of every range of
+    HGAP_RANGE_BYTES input bytes (int32 tensor of range_count(n) x 256) for encode()."""
+    d_in = _as_device_u8(d_in)
+    n = d_in.numel()
+    hist = torch.empty(256, dtype=torch.int32, device=d_in.device)
+    rows = int(_lib().hgap_range_count(n))
+    if range_hist is None or range_hist.numel() < 256 * rows:
+        range_hist = torch.empty((max(rows, 1), 256), dtype=torch.int32, device=d_in.device)
+    check(_lib().hgap_histogram_ranges(_ptr(d_in), n, _ptr(hist), _ptr(range_hist),
+                                       _stream(stream)), "hgap_histogram_ranges")
+    return hist, range_hist
+
+
 def host_histogram(hist):
     """Counts of a histogram tensor on the host as uint32 (the device tensor is int32: counts of
     2^31 and more read back as negative numbers unless reinterpreted)."""
@@ -175,14 +189,17 @@ def _cfg_ref(config):
 
 
 def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, payload=None,
-           gap=None, config=None):
+           gap=None, config=None, range_hist=None):
     """Encode a device uint8 buffer.  Without `cb`, builds the histogram and codebook first
-    (device histogram, host codebook) like the reference's main (encoder/src/huff.cpp:69-99)."""
+    (device histogram, host codebook) like the reference's main (encoder/src/huff.cpp:69-99).
+    With `range_hist` (from histogram_ranges over the same input) the encoder derives every
+    range's bit offset from the counts and runs without waiting between tiles; the stream is
+    the same."""
     d_in = _as_device_u8(d_in)
     n = d_in.numel()
     if cb is None or (sizes is None and hist is None):
         if hist is None:
-            hist = histogram(d_in, stream)
+            hist, range_hist = histogram_ranges(d_in, stream)
         hist = _hist_array(hist)
         if cb is None:
             cb = codebook_build(hist)
@@ -195,7 +212,12 @@ def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, pa
         gap = _words(sizes.gap_words, dev)
     wsb = _lib().hgap_encode_workspace_bytes(n)
     ws = (workspace or Workspace(dev)).get(wsb)
-    if config is None:  # default shape, falling back to smaller tiles when it does not fit
+    if range_hist is not None:
+        check(_lib().hgap_encode_ranges(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
+                                        _ptr(range_hist), _ptr(payload), _ptr(gap), _ptr(ws),
+                                        ws.numel(), _cfg_ref(config), _stream(stream)),
+              "hgap_encode_ranges")
+    elif config is None:  # default shape, falling back to smaller tiles when it does not fit
         check(_lib().hgap_encode(_ptr(d_in), n, ctypes.byref(cb), ctypes.byref(sizes),
                                  _ptr(payload), _ptr(gap), _ptr(ws), ws.numel(), _stream(stream)),
               "hgap_encode")

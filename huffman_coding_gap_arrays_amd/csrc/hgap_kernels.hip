@@ -110,14 +110,15 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// `published`: the tile's aggregate is already in its status word (stored by an earlier launch).
 __device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregate,
-                              uint32_t wpl) {
+                              uint32_t wpl, bool published = false) {
     const uint32_t lane = lane_id();
     if (tile == 0) {
         if (lane == 0) status_store(&status[0], kFlagP | aggregate);
         return 0;
     }
-    if (lane == 0) status_store(&status[tile], kFlagA | aggregate);
+    if (lane == 0 && !published) status_store(&status[tile], kFlagA | aggregate);
     uint64_t excl = 0;
     int64_t end = (int64_t)tile - 1;
     while (true) {
@@ -240,6 +241,64 @@ __global__ __launch_bounds__(kHistThreadsSingle) void hist_kernel(const uint8_t 
     }
 }
 
+// The same histogram, plus one histogram per range of `range_bytes` consecutive input bytes
+// (range r = bytes [r * range_bytes, (r + 1) * range_bytes), the last one ragged).  A workgroup
+// counts whole ranges (ranges blockIdx.x, blockIdx.x + gridDim.x, ...); after each one,
+// thread b < 256 sums bin b over the 64 copies, clears them as it reads, and stores the range's
+// count; it keeps a running total of bin b for the one global atomic at the end.
+__global__ __launch_bounds__(kHistThreadsSingle) void hist_range_kernel(
+    const uint8_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ hist,
+    uint32_t *__restrict__ range_hist, uint32_t ranges, uint32_t range_bytes) {
+    constexpr int kThreads = kHistThreadsSingle;
+    __shared__ uint32_t s_bins[kHistCopies * 256];
+    for (int i = threadIdx.x; i < kHistCopies * 256; i += kThreads) s_bins[i] = 0u;
+    __syncthreads();
+    uint32_t *mine = s_bins + lane_id();
+    const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
+    uint32_t total = 0;
+    for (uint32_t r = blockIdx.x; r < ranges; r += gridDim.x) {
+        const uint64_t lo = (uint64_t)r * range_bytes;
+        const uint64_t hi = lo + range_bytes < n ? lo + range_bytes : n;
+        const uint64_t v1 = hi >> 4;  // lo is a multiple of 16
+        uint64_t v = (lo >> 4) + threadIdx.x;
+        for (; v + 3 * kThreads < v1; v += 4 * kThreads) {
+            u32x4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) q[u] = __builtin_nontemporal_load(&in4[v + u * kThreads]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
+            }
+        }
+        for (; v < v1; v += kThreads) {
+            const u32x4 q = __builtin_nontemporal_load(&in4[v]);
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
+        }
+        // tail bytes (n % 16) of the last range
+        if (hi == n) {
+            const uint64_t t = (v1 << 4) + threadIdx.x;
+            if (t < n) atomicAdd(&mine[(uint32_t)in[t] << 6], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            uint32_t sum = 0;
+            for (int c = 0; c < kHistCopies; c++) {
+                const uint32_t i = (threadIdx.x << 6) + ((c + threadIdx.x) & 63u);
+                sum += s_bins[i];
+                s_bins[i] = 0u;
+            }
+            range_hist[256ull * r + threadIdx.x] = sum;
+            total += sum;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 256 && total) atomicAdd(&hist[threadIdx.x], total);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Encoder (replaces cuencoder + cu_get_gaparray, encoder/src/encoder.cu:142-379).
 //
@@ -357,11 +416,30 @@ __device__ __forceinline__ EncStage<K> enc_read_stage(const uint8_t *s_stage) {
     return st;
 }
 
-template <int THREADS, int K>
+// How a chained encoder finds E.  The input is cut into ranges of whole tiles; a workgroup
+// encodes a range's tiles in order.  The first tile of a range looks back over per-range status
+// words whose aggregates (the ranges' bit counts, from per-range histograms) were all stored
+// before the launch, so it never waits on another workgroup; it also claims the workgroup's
+// next range.  A later tile's E is the previous tile's E plus its bit count, and the previous
+// tile's last partial word (`carry`) supplies the bits of E's word before E, so the tile does
+// not rebuild them from input bytes.
+struct EncChain {
+    uint32_t range;            // range of the current tile
+    uint32_t next_range;       // claimed by the current range's first tile
+    uint32_t ranges;
+    uint32_t tiles_per_range;
+    uint32_t end;              // one past the current range's last tile
+    bool first;                // the current tile is its range's first
+    uint64_t E;                // in: E of the current tile (when !first); out: the next tile's
+    uint32_t carry;            // in: bits of E's word before E (when !first); out: the next's
+};
+
+template <int THREADS, int K, bool CHAINED>
 __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
                                          const EncStage<K> &st, const uint32_t *s_tab,
                                          uint32_t *s_out, uint32_t *s_gap, uint32_t *s_wave,
-                                         uint64_t *s_excl, uint32_t wpl, NextTile &next) {
+                                         uint64_t *s_excl, uint32_t wpl, NextTile &next,
+                                         EncChain &ch) {
     constexpr uint32_t TILE = THREADS * K;
     const uint32_t lane = lane_id();
     const uint8_t *in = buf.in;
@@ -386,22 +464,39 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         }
     }
 
-    // 2. tile scan, zero the used part of the LDS images, look-back
+    // 2. tile scan, zero the used part of the LDS images (a chained tile that knows E puts the
+    // carried bits into its first word instead), look-back
     uint32_t tile_bits;
     const uint32_t excl = block_excl_sum<THREADS>(bits, s_wave, tile_bits);
     const uint32_t used_words = (tile_bits >> 5) + 8, used_gap = (tile_bits >> 9) + 5;
-    for (uint32_t i = threadIdx.x; i < used_words; i += THREADS) s_out[i] = 0u;
+    const bool known = CHAINED && !ch.first;
+    const uint32_t i0 = known ? (uint32_t)((ch.E >> 5) & 3u) : 0xFFFFFFFFu;
+    const uint32_t carry = known ? ch.carry : 0u;
+    for (uint32_t i = threadIdx.x; i < used_words; i += THREADS) s_out[i] = i == i0 ? carry : 0u;
     for (uint32_t i = threadIdx.x; i < used_gap; i += THREADS) s_gap[i] = 0u;
-    if (threadIdx.x < 64) {
-        const uint64_t E = look_back(buf.status, tile, tile_bits, wpl);
+    if (!known && threadIdx.x < 64) {
+        uint64_t E;
+        if (CHAINED) {
+            const uint64_t agg = status_load(&buf.status[ch.range]) & kValueMask;
+            E = look_back(buf.status, ch.range, agg, wpl, true);
+        } else {
+            E = look_back(buf.status, tile, tile_bits, wpl);
+        }
         if (threadIdx.x == 0) {
             *s_excl = E;
             if (next.counter) *next.s_ticket = atomicAdd(next.counter, 1u);
         }
     }
     __syncthreads();
-    const uint64_t E = *s_excl;
-    if (next.counter) {
+    const uint64_t E = known ? ch.E : *s_excl;
+    if (CHAINED) {
+        if (ch.first) ch.next_range = *next.s_ticket;
+        next.tile = tile + 1 < ch.end ? tile + 1
+                    : (ch.next_range < ch.ranges ? ch.next_range * ch.tiles_per_range
+                                                 : buf.num_tiles);
+        if (next.s_stage && enc_tile_full<THREADS, K>(buf, next.tile))
+            enc_stage_async<THREADS, K>(buf, next.tile, next.s_stage);
+    } else if (next.counter) {
         next.tile = *next.s_ticket;
         if (next.s_stage && enc_tile_full<THREADS, K>(buf, next.tile))
             enc_stage_async<THREADS, K>(buf, next.tile, next.s_stage);
@@ -444,25 +539,33 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     }
     __syncthreads();
 
-    // 4a. first word: add the previous tile's last E%32 bits
+    // 4a. first word: add the previous tile's last E%32 bits (a chained tile that knows E has
+    // them already)
     const uint32_t r = (uint32_t)(E & 31);
-    if (threadIdx.x < 64 && r != 0) {
-        uint32_t l = 0, c = 0;
-        if (lane < 32) {
-            const uint32_t ee = s_tab[in[tile_start - 1 - lane]];
-            l = ee & 31u;
-            c = ee >> 5;
+    if (!known && r != 0) {
+        if (threadIdx.x < 64) {
+            uint32_t l = 0, c = 0;
+            if (lane < 32) {
+                const uint32_t ee = s_tab[in[tile_start - 1 - lane]];
+                l = ee & 31u;
+                c = ee >> 5;
+            }
+            const uint32_t cum = wave_incl_sum(l);
+            const uint32_t after = cum - l;  // bits between this code's end and the tile start
+            const uint32_t part = (lane < 32 && after < r) ? (c << (32 - (r - after))) : 0u;
+            const uint32_t prev_bits = wave_or_u32(part);
+            if (threadIdx.x == 0) s_out[(E >> 5) - base_w] |= prev_bits;
         }
-        const uint32_t cum = wave_incl_sum(l);
-        const uint32_t after = cum - l;  // bits between this code's end and the tile start
-        const uint32_t part = (lane < 32 && after < r) ? (c << (32 - (r - after))) : 0u;
-        const uint32_t prev_bits = wave_or_u32(part);
-        if (threadIdx.x == 0) s_out[(E >> 5) - base_w] |= prev_bits;
+        __syncthreads();
     }
-    __syncthreads();
+    const uint64_t T = tile_bits;
+    if (CHAINED) {  // the word holding the tile's end is complete up to the end: carry it
+        const uint64_t end_bit = E + T;
+        ch.carry = (end_bit & 31u) ? s_out[(end_bit >> 5) - base_w] : 0u;
+        ch.E = end_bit;
+    }
 
     // 4b. payload words
-    const uint64_t T = tile_bits;
     const uint64_t own_lo = E >> 5;
     uint64_t own_hi = (tile + 1 == buf.num_tiles) ? ((E + T + 31) >> 5) : ((E + T) >> 5);
     if (own_hi > buf.payload_words) own_hi = buf.payload_words;
@@ -507,11 +610,13 @@ __host__ __device__ constexpr uint32_t enc_stage_word(uint32_t out_words, uint32
 
 // STAGE: the next tile's input is copied into LDS (a tile's worth of bytes after the images)
 // while the current tile packs and stores, claimed right after the current tile's look-back.
-template <int THREADS, int K, bool STAGE>
+// CHAINED: the counter hands out ranges of `tiles_per_range` tiles (buf.status has one word per
+// range, prepared by enc_range_prep_kernel); otherwise single tiles.
+template <int THREADS, int K, bool STAGE, bool CHAINED>
 __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable table,
                                                       uint32_t *__restrict__ counter,
                                                       uint32_t out_words, uint32_t stage_word,
-                                                      uint32_t wpl) {
+                                                      uint32_t wpl, uint32_t tiles_per_range) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_tab[256];
     __shared__ uint32_t s_wave[THREADS / 64];
@@ -523,16 +628,65 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
     for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = table.e[i];
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
-    NextTile next{counter, &s_tile, s_tile, STAGE ? s_stage : nullptr};
+    EncChain ch{};
+    uint32_t first_tile = s_tile;
+    if (CHAINED) {
+        ch.ranges = (buf.num_tiles + tiles_per_range - 1) / tiles_per_range;
+        ch.tiles_per_range = tiles_per_range;
+        ch.next_range = s_tile;
+        first_tile = s_tile < ch.ranges ? s_tile * tiles_per_range : buf.num_tiles;
+    }
+    NextTile next{counter, &s_tile, first_tile, STAGE ? s_stage : nullptr};
     if (STAGE && enc_tile_full<THREADS, K>(buf, next.tile))
         enc_stage_async<THREADS, K>(buf, next.tile, s_stage);
+    if (CHAINED) {
+        while (ch.next_range < ch.ranges) {
+            ch.range = ch.next_range;
+            const uint32_t lo = ch.range * tiles_per_range;
+            ch.end = lo + tiles_per_range < buf.num_tiles ? lo + tiles_per_range : buf.num_tiles;
+            for (uint32_t cur = lo; cur < ch.end; cur++) {
+                ch.first = cur == lo;
+                const EncStage<K> st = (STAGE && enc_tile_full<THREADS, K>(buf, cur))
+                                           ? enc_read_stage<THREADS, K>(s_stage)
+                                           : enc_fetch<THREADS, K>(buf, cur);
+                enc_tile<THREADS, K, true>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl,
+                                           wpl, next, ch);
+            }
+        }
+        return;
+    }
     while (next.tile < buf.num_tiles) {
         const uint32_t cur = next.tile;
         const EncStage<K> st = (STAGE && enc_tile_full<THREADS, K>(buf, cur))
                                    ? enc_read_stage<THREADS, K>(s_stage)
                                    : enc_fetch<THREADS, K>(buf, cur);
-        enc_tile<THREADS, K>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl, next);
+        enc_tile<THREADS, K, false>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
+                                    next, ch);
     }
+}
+
+// Before a chained encode: zero the gap array and the range counter, and store every range's
+// bit count (its histogram dotted with the code lengths) in its status word as a look-back
+// aggregate (range 0's as its inclusive prefix).  One wave per range; lane l reads bins
+// 4l..4l+3.
+__global__ __launch_bounds__(256) void enc_range_prep_kernel(
+    const uint32_t *__restrict__ range_hist, uint32_t ranges, EncTable table,
+    uint64_t *__restrict__ status, uint32_t *__restrict__ counter, uint32_t *__restrict__ gap,
+    uint32_t gap_words) {
+    const uint32_t lane = lane_id();
+    uint32_t len[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) len[j] = table.e[4 * lane + j] & 31u;
+    const uint32_t waves = gridDim.x * 4u;
+    for (uint32_t r = blockIdx.x * 4u + (threadIdx.x >> 6); r < ranges; r += waves) {
+        const u32x4 h = *reinterpret_cast<const u32x4 *>(range_hist + 256ull * r + 4 * lane);
+        const uint64_t bits = wave_sum_u64((uint64_t)h[0] * len[0] + (uint64_t)h[1] * len[1] +
+                                           (uint64_t)h[2] * len[2] + (uint64_t)h[3] * len[3]);
+        if (lane == 0) status_store(&status[r], (r == 0 ? kFlagP : kFlagA) | (bits & kValueMask));
+    }
+    const uint32_t stride = gridDim.x * 256u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < gap_words; i += stride) gap[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1098,8 +1252,9 @@ __global__ __launch_bounds__(THREADS) void enc_batch_kernel(
         buf.reserved = 0;
         const EncStage<K> st = enc_fetch<THREADS, K>(buf, g - job.base);
         NextTile none{nullptr, nullptr, 0u, nullptr};
-        enc_tile<THREADS, K>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
-                             none);
+        EncChain unused{};
+        enc_tile<THREADS, K, false>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl,
+                                    wpl, none, unused);
     }
 }
 
@@ -1344,35 +1499,50 @@ static uint32_t window_per_lane(const hgap_launch_config &c) {
     return w < 1 ? 1 : (w > kMaxWindowPerLane ? kMaxWindowPerLane : w);
 }
 
-template <int THREADS, int K>
-static hgap_status launch_enc_t(const EncBuffer &b, const EncTable &t, uint32_t *counter,
-                                uint32_t max_len, const hgap_launch_config &c, hipStream_t s) {
+// tiles_per_range = 0: one tile per ticket and a look-back per tile; otherwise the chained
+// encoder over ranges of that many tiles
+template <int THREADS, int K, bool CHAINED>
+static hgap_status launch_enc_kernel(const EncBuffer &b, const EncTable &t, uint32_t *counter,
+                                     uint32_t max_len, const hgap_launch_config &c,
+                                     uint32_t tiles_per_range, hipStream_t s) {
     constexpr uint32_t tile = THREADS * K;
     const uint32_t out_words = enc_out_words(tile, max_len);
     const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
+    const uint64_t units =
+        CHAINED ? (b.num_tiles + tiles_per_range - 1) / tiles_per_range : b.num_tiles;
     // with the input staging area when it fits the CU's LDS, else without
     const uint32_t stage_word = enc_stage_word(out_words, enc_gap_words(tile, max_len));
     const size_t lds_staged = 4ull * stage_word + tile;
-    auto staged = enc_kernel<THREADS, K, true>;
+    auto staged = enc_kernel<THREADS, K, true, CHAINED>;
     if (allow_dynamic_lds(staged, lds_staged)) {
-        const unsigned grid = grid_for(staged, THREADS, lds_staged, c.blocks_per_cu, b.num_tiles);
+        const unsigned grid = grid_for(staged, THREADS, lds_staged, c.blocks_per_cu, units);
         hipLaunchKernelGGL(staged, dim3(grid), dim3(THREADS), lds_staged, s, b, t, counter,
-                           out_words, stage_word, window_per_lane(c));
+                           out_words, stage_word, window_per_lane(c), tiles_per_range);
         return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
     }
-    auto kern = enc_kernel<THREADS, K, false>;
+    auto kern = enc_kernel<THREADS, K, false, CHAINED>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
-    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, b.num_tiles);
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, units);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words, 0u,
-                       window_per_lane(c));
+                       window_per_lane(c), tiles_per_range);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
+template <int THREADS, int K>
+static hgap_status launch_enc_t(const EncBuffer &b, const EncTable &t, uint32_t *counter,
+                                uint32_t max_len, const hgap_launch_config &c,
+                                uint32_t tiles_per_range, hipStream_t s) {
+    if (tiles_per_range)
+        return launch_enc_kernel<THREADS, K, true>(b, t, counter, max_len, c, tiles_per_range, s);
+    return launch_enc_kernel<THREADS, K, false>(b, t, counter, max_len, c, 0u, s);
+}
+
 static hgap_status launch_enc(const EncBuffer &b, const EncTable &t, uint32_t *counter,
-                              uint32_t max_len, const hgap_launch_config &c, hipStream_t s) {
+                              uint32_t max_len, const hgap_launch_config &c,
+                              uint32_t tiles_per_range, hipStream_t s) {
 #define HGAP_ENC_CASE(T, K)                                                      \
     if (c.enc_threads == T && c.enc_bytes_per_thread == K)                     \
-        return launch_enc_t<T, K>(b, t, counter, max_len, c, s);
+        return launch_enc_t<T, K>(b, t, counter, max_len, c, tiles_per_range, s);
     HGAP_ENC_CASE(256, 16) HGAP_ENC_CASE(256, 32) HGAP_ENC_CASE(256, 64)
     HGAP_ENC_CASE(512, 16) HGAP_ENC_CASE(512, 32) HGAP_ENC_CASE(512, 64)
     HGAP_ENC_CASE(1024, 16) HGAP_ENC_CASE(1024, 32) HGAP_ENC_CASE(1024, 64)
@@ -1624,7 +1794,7 @@ template <int THREADS, int K>
 static bool enc_shape_fits(uint32_t max_len) {
     constexpr uint32_t tile = THREADS * K;
     const size_t lds = 4ull * (enc_out_words(tile, max_len) + enc_gap_words(tile, max_len));
-    return lds_fits(enc_kernel<THREADS, K, false>, lds) &&
+    return lds_fits(enc_kernel<THREADS, K, false, false>, lds) &&
            lds_fits(enc_batch_kernel<THREADS, K>, lds);
 }
 
@@ -1695,10 +1865,35 @@ size_t hgap_encode_workspace_bytes(uint64_t n) {
     return 16 + 8 * tiles;
 }
 
-hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *cb,
-                            const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
-                            void *d_workspace, size_t workspace_bytes,
-                            const hgap_launch_config *cfg, hgap_stream_t stream) {
+uint32_t hgap_range_count(uint64_t n) {
+    return (uint32_t)((n + HGAP_RANGE_BYTES - 1) / HGAP_RANGE_BYTES);
+}
+
+hgap_status hgap_histogram_ranges(const void *d_in, uint64_t n, uint32_t *d_hist,
+                                  uint32_t *d_range_hist, hgap_stream_t stream) {
+    if (!d_hist || (n > 0 && (!d_in || !d_range_hist))) return HGAP_ERR_INVALID_ARG;
+    if (!aligned16(d_in) || !aligned16(d_hist) || !aligned16(d_range_hist))
+        return HGAP_ERR_INVALID_ARG;
+    if (n > 0xFFFFFFFFull) return HGAP_ERR_SIZE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hgap_status st = launch_zero(d_hist, 256, nullptr, 0, s);
+    if (st != HGAP_OK || n == 0) return st;
+    const uint32_t ranges = hgap_range_count(n);
+    const unsigned blocks = grid_for(hist_range_kernel, kHistThreadsSingle, 0, 0, ranges);
+    hipLaunchKernelGGL(hist_range_kernel, dim3(blocks), dim3(kHistThreadsSingle), 0, s,
+                       static_cast<const uint8_t *>(d_in), n, d_hist, d_range_hist, ranges,
+                       (uint32_t)HGAP_RANGE_BYTES);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+static_assert(HGAP_RANGE_BYTES % (1024u * 64u) == 0, "ranges hold whole tiles of every shape");
+
+// hgap_encode_cfg, and with d_range_hist the chained encoder
+static hgap_status encode_common(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                                 const hgap_sizes *sizes, const uint32_t *d_range_hist,
+                                 uint32_t *d_payload, uint32_t *d_gap, void *d_workspace,
+                                 size_t workspace_bytes, const hgap_launch_config *cfg,
+                                 hgap_stream_t stream) {
     if (!cb || !sizes) return HGAP_ERR_INVALID_ARG;
     const hgap_launch_config c = resolve(cfg);
     if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
@@ -1706,7 +1901,7 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
                                                                             : HGAP_ERR_SIZE;
     if (!d_in || !d_payload || !d_gap || !d_workspace) return HGAP_ERR_INVALID_ARG;
     if (!aligned16(d_in) || !aligned16(d_payload) || !aligned16(d_gap) ||
-        !aligned16(d_workspace))
+        !aligned16(d_workspace) || !aligned16(d_range_hist))
         return HGAP_ERR_INVALID_ARG;
     if (n > 0xFFFFFFFFull) return HGAP_ERR_SIZE;
     if (workspace_bytes < hgap_encode_workspace_bytes(n)) return HGAP_ERR_WORKSPACE;
@@ -1726,8 +1921,23 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
     const uint32_t gap_words = (uint32_t)(((uint64_t)sizes->gap_elements + 7) / 8);
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t *ws = static_cast<uint32_t *>(d_workspace);
-    hgap_status st = launch_zero(d_gap, gap_words, ws, 4 + 2 * tiles, s);
-    if (st != HGAP_OK) return st;
+    uint64_t *status = reinterpret_cast<uint64_t *>(ws + 4);
+    uint32_t tiles_per_range = 0;
+    if (d_range_hist) {
+        // the prep kernel zeroes the gap array and the counter and fills one status word per
+        // range (the workspace holds one per tile, at least as many)
+        tiles_per_range = (uint32_t)(HGAP_RANGE_BYTES / tile);
+        const uint32_t ranges = hgap_range_count(n);
+        (void)hipGetLastError();
+        uint64_t blocks = std::max<uint64_t>((ranges + 3) / 4, (gap_words + 255) / 256);
+        blocks = std::min<uint64_t>(std::max<uint64_t>(blocks, 1), 2048);
+        hipLaunchKernelGGL(enc_range_prep_kernel, dim3((unsigned)blocks), dim3(256), 0, s,
+                           d_range_hist, ranges, t, status, ws, d_gap, gap_words);
+        if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
+    } else {
+        hgap_status st = launch_zero(d_gap, gap_words, ws, 4 + 2 * tiles, s);
+        if (st != HGAP_OK) return st;
+    }
     EncBuffer b;
     b.in = static_cast<const uint8_t *>(d_in);
     b.n = n;
@@ -1735,10 +1945,40 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
     b.payload_words = sizes->payload_words;
     b.gap_words = gap_words;
     b.gap = d_gap;
-    b.status = reinterpret_cast<uint64_t *>(ws + 4);
+    b.status = status;
     b.num_tiles = (uint32_t)tiles;
     b.reserved = 0;
-    return launch_enc(b, t, ws, max_len, c, s);
+    return launch_enc(b, t, ws, max_len, c, tiles_per_range, s);
+}
+
+hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                            const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
+                            void *d_workspace, size_t workspace_bytes,
+                            const hgap_launch_config *cfg, hgap_stream_t stream) {
+    return encode_common(d_in, n, cb, sizes, nullptr, d_payload, d_gap, d_workspace,
+                         workspace_bytes, cfg, stream);
+}
+
+// cfg NULL: the default shape first, then smaller tiles when its LDS image does not fit
+hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                               const hgap_sizes *sizes, const uint32_t *d_range_hist,
+                               uint32_t *d_payload, uint32_t *d_gap, void *d_workspace,
+                               size_t workspace_bytes, const hgap_launch_config *cfg,
+                               hgap_stream_t stream) {
+    if (n > 0 && !d_range_hist) return HGAP_ERR_INVALID_ARG;
+    if (cfg)
+        return encode_common(d_in, n, cb, sizes, d_range_hist, d_payload, d_gap, d_workspace,
+                             workspace_bytes, cfg, stream);
+    hgap_launch_config c;
+    hgap_default_config(&c);
+    hgap_status st = HGAP_ERR_UNSUPPORTED;
+    for (uint32_t threads = c.enc_threads; threads >= 256; threads /= 2) {
+        c.enc_threads = threads;
+        st = encode_common(d_in, n, cb, sizes, d_range_hist, d_payload, d_gap, d_workspace,
+                           workspace_bytes, &c, stream);
+        if (st != HGAP_ERR_UNSUPPORTED) break;
+    }
+    return st;
 }
 
 // Default shape first; when its LDS image does not fit (long codes), smaller tiles.

@@ -148,6 +148,26 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
                             void *d_workspace, size_t workspace_bytes,
                             const hgap_launch_config *cfg, hgap_stream_t stream);
 
+/* The histogram and encode pair of the reference's pipeline (histgram() then encode(),
+ * encoder/src/huff.cpp:69-99) with per-range counts handed from one to the other.  Range r is
+ * input bytes [r * HGAP_RANGE_BYTES, (r + 1) * HGAP_RANGE_BYTES), the last one ragged.
+ * hgap_histogram_ranges writes d_hist[256] as hgap_histogram does, plus d_range_hist[256 * r
+ * + b] = count of byte b in range r (hgap_range_count(n) x 256 u32).  hgap_encode_ranges
+ * writes the same payload and gap array as hgap_encode; it takes d_range_hist from
+ * hgap_histogram_ranges over the same input (other counts give a wrong stream, never an access
+ * outside the caller's buffers), derives every range's bit offset from them before encoding,
+ * and so needs no tile to wait for another.  Workspace: hgap_encode_workspace_bytes(n).
+ * cfg NULL = the default shape, falling back to smaller tiles like hgap_encode. */
+#define HGAP_RANGE_BYTES (256u * 1024u)
+uint32_t hgap_range_count(uint64_t n);
+hgap_status hgap_histogram_ranges(const void *d_in, uint64_t n, uint32_t *d_hist,
+                                  uint32_t *d_range_hist, hgap_stream_t stream);
+hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook *cb,
+                               const hgap_sizes *sizes, const uint32_t *d_range_hist,
+                               uint32_t *d_payload, uint32_t *d_gap, void *d_workspace,
+                               size_t workspace_bytes, const hgap_launch_config *cfg,
+                               hgap_stream_t stream);
+
 /* Decoder workspace: tile counter + one 64-bit look-back status word per tile of segments. */
 size_t hgap_decode_workspace_bytes(uint32_t gap_elements);
 /* Decodes n bytes into d_out from payload[W] + gap array using a table image already in device

@@ -80,6 +80,75 @@ def test_histogram(codec, n):
     assert (h.cpu().numpy().astype(np.int64) == want).all()
 
 
+RANGE = 256 * 1024  # HGAP_RANGE_BYTES
+RANGE_SIZES = [1, 15, 16, 17, 4095, 65536, RANGE - 1, RANGE, RANGE + 1, 3 * RANGE - 17,
+               (5 << 20) + 123]
+
+
+def range_counts(d, n):
+    rows = -(-n // RANGE)
+    want = np.zeros((rows, 256), dtype=np.int64)
+    for r in range(rows):
+        want[r] = np.bincount(d[r * RANGE: (r + 1) * RANGE], minlength=256)
+    return want
+
+
+@pytest.mark.parametrize("dist", ["zipf", "skew", "fib", "k1"])
+def test_histogram_ranges_and_chained_encode(codec, orc, dist):
+    """Per-range counts equal numpy's; the encoder that takes them writes the oracle's payload
+    and gap array bit for bit, for ragged and whole ranges, and its stream decodes."""
+    cdf, perm = dist_spec(dist)
+    for n in RANGE_SIZES:
+        say("ranges", dist, "n =", n)
+        d_dev = device_data(codec, n, 5 + n, cdf, perm)
+        d = d_dev.cpu().numpy()
+        h, rh = codec.histogram_ranges(d_dev)
+        rows = int(codec._lib().hgap_range_count(n))
+        assert rows == -(-n // RANGE)
+        assert (codec.host_histogram(h).astype(np.int64) == np.bincount(d, minlength=256)).all()
+        got = rh.cpu().numpy().view(np.uint32).reshape(-1, 256)[:rows].astype(np.int64)
+        assert (got == range_counts(d, n)).all(), f"{dist} n={n}: range counts"
+        hist = orc.histogram(d)
+        ocb = orc.codebook(hist)
+        pay, gap = orc.encode(d, ocb)
+        enc = codec.encode(d_dev, hist=hist, range_hist=rh)
+        assert (enc.payload_u32() == pay).all(), f"{dist} n={n}: payload"
+        assert (enc.gap_u32() == gap).all(), f"{dist} n={n}: gap"
+        table = codec.DeviceTable(enc.codebook)
+        out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
+        assert (out.cpu().numpy() == d).all(), f"{dist} n={n}: round trip"
+
+
+def test_chained_encode_every_shape(codec, orc):
+    """Every encoder shape (tiles of 4 KiB to 64 KiB, 64 down to 4 per range) writes the same
+    stream through the chained encoder; repeated calls reuse one workspace."""
+    cdf, perm = dist_spec("zipf")
+    n = 3 * RANGE + 4097
+    d_dev = device_data(codec, n, 21, cdf, perm)
+    d = d_dev.cpu().numpy()
+    hist = orc.histogram(d)
+    pay, gap = orc.encode(d, orc.codebook(hist))
+    _, rh = codec.histogram_ranges(d_dev)
+    cb = codec.codebook_build(hist)
+    sizes = codec.encode_sizes(cb, hist)
+    ws = codec.Workspace()
+    ran = 0
+    for threads in (256, 512, 1024):
+        for k in (16, 32, 64):
+            cfg = {"enc_threads": threads, "enc_bytes_per_thread": k}
+            say("chained shape", cfg)
+            try:
+                enc = codec.encode(d_dev, cb=cb, sizes=sizes, range_hist=rh, config=cfg,
+                                   workspace=ws)
+            except codec.HgapError as e:
+                assert e.status == 8, cfg  # LDS image too large for this shape and codebook
+                continue
+            ran += 1
+            assert (enc.payload_u32() == pay).all(), cfg
+            assert (enc.gap_u32() == gap).all(), cfg
+    assert ran >= 8
+
+
 def check_encode(codec, orc, d_dev, label):
     d = d_dev.cpu().numpy()
     hist = orc.histogram(d)
@@ -232,6 +301,14 @@ def test_full_size_configs(codec, orc, golden_codebooks, name):
     table = codec.DeviceTable(enc.codebook)
     out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
     assert torch.equal(out, d_dev), "round trip"
+    say("  chained encode", name)
+    h2, rh = codec.histogram_ranges(d_dev)
+    assert (codec.host_histogram(h2).astype(np.uint64) == h).all()
+    W, GW = enc.payload_words, int(enc.sizes.gap_words)
+    enc2 = codec.encode(d_dev, cb=enc.codebook, sizes=enc.sizes, range_hist=rh)
+    assert torch.equal(enc2.payload[:W], enc.payload[:W]), "chained payload"
+    assert torch.equal(enc2.gap[:GW], enc.gap[:GW]), "chained gap"
+    del enc2, rh
     say("  oracle encode", name)
     d = d_dev.cpu().numpy()
     del out

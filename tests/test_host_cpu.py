@@ -240,6 +240,21 @@ def test_device_entry_points_validate_before_launch(capi):
     assert capi.hgap_decode(p, 8, p, 3, p, ctypes.byref(info), p, 10, p, 4096, None) == 2
     assert capi.hgap_decode(p, 8, p, 2, p, ctypes.byref(info), p, 10, p, 8, None) == 4
     assert capi.hgap_histogram(ctypes.c_void_p(3), 10, ctypes.c_void_p(16), None) == 1
+    # per-range histograms and the encoder that takes them
+    R = 256 * 1024
+    assert [capi.hgap_range_count(x) for x in (0, 1, R, R + 1, (1 << 32) - 1)] == \
+        [0, 1, 1, 2, 1 << 14]
+    q = ctypes.c_void_p(16)
+    assert capi.hgap_histogram_ranges(q, 10, None, q, None) == 1
+    assert capi.hgap_histogram_ranges(q, 10, q, None, None) == 1
+    assert capi.hgap_histogram_ranges(q, 10, q, ctypes.c_void_p(20), None) == 1
+    assert capi.hgap_histogram_ranges(q, 1 << 32, q, q, None) == 2
+    assert capi.hgap_encode_ranges(q, 100, ctypes.byref(cb), ctypes.byref(sz), None, q, q, q,
+                                   4096, None, None) == 1  # no range counts
+    assert capi.hgap_encode_ranges(q, 100, ctypes.byref(cb), ctypes.byref(sz),
+                                   ctypes.c_void_p(24), q, q, q, 4096, None, None) == 1
+    assert capi.hgap_encode_ranges(q, 100, ctypes.byref(cb), ctypes.byref(sz), q, q, q, q, 8,
+                                   None, None) == 4  # workspace too small
     assert capi.hgap_encode_workspace_bytes(0) == 16
     assert capi.hgap_encode_workspace_bytes(8193) == 16 + 3 * 8  # smallest tile: 4 KiB
     # ticket + status word of one offsets block, one count byte per segment (whole 1024-segment
