@@ -266,16 +266,15 @@ def main():
                             uncompressed_GBps=round(n / v["seconds"] / 1e9, 2),
                             alg_GBps=round(gbs, 2), roofline_frac=round(gbs / HBM_PEAK_GBS, 4))
     per_stage["codebook_host"] = dict(ms=round(t_book * 1e3, 4))
-    # the slower of encode (one kernel) and decode (count, offsets and write kernels after a
-    # workspace fill, timed as one unit between events on their stream)
+    # the slower of encode (range prep and encoder kernels) and decode (count, offsets and write
+    # kernels after a workspace fill), each timed as one unit between events on their stream
     dom = max(("encode", "decode"), key=lambda k: stages[k]["seconds"])
-    kern = {"encode": "enc_kernel",
+    kern = {"encode": "enc_range_prep_kernel+enc_kernel",
             "decode": "dec_count_kernel+dec_offsets_kernel+dec_write_kernel"}[dom]
     achieved = stages[dom]["alg_bytes"] / stages[dom]["seconds"] / 1e9
     roofline = dict(bound="hbm", kernel=kern, achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=load_traffic(args.config, {"encode": "enc_kernel",
-                                                       "decode": "decode"}[dom]),
+                    traffic=load_traffic(args.config, dom),
                     alg_bytes_per_launch=stages[dom]["alg_bytes"])
     batched = None
     if args.batch_count > 0:

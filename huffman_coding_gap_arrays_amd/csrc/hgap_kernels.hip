@@ -1056,11 +1056,23 @@ __device__ __forceinline__ void dec_write_tile(const DecBuffer &buf, DecTable t,
     const uint32_t a = lead + excl;  // this lane's first image byte
     const uint32_t total = lead + tile_syms;
     if (total + 17u <= image_bytes) {  // uniform
-        // walk 2 into the image; after a barrier every lane writes its first symbol again
-        // over the stray byte its predecessor left there
-        if (cnt) dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
-        __syncthreads();
-        if (cnt) s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+        // walk 2 into the image.  Every segment of the stream has a codeword, so the stray
+        // byte on a lane's first byte comes from the lane before it.  One wave's LDS stores
+        // land in program order: a lane writing its first symbol again after its walk lands
+        // after its predecessor's stray byte when both are in this wave, and lane 63 writes
+        // its successor's first symbol (decoded from the next segment's first word and gap
+        // nibble) after its own stray byte, for the lane 0 of the next wave.
+        if (cnt) {
+            dec_write<SINGLE>(g, s_tab, P, t.l1_entries, DecLdsOut{s_img + a});
+            s_img[a] = dec_first<SINGLE>(g, s_tab, P, t.l1_entries);
+            if (lane_id() == 63u) {
+                // element `seg` shares lane 63's gap word (seg % 8 == 7)
+                DecSeg nx{};
+                nx.w0 = g.w4;
+                nx.p0 = (st.gapw >> ((seg & 7u) * 4u)) & 15u;
+                s_img[a + cnt] = dec_first<SINGLE>(nx, s_tab, P, t.l1_entries);
+            }
+        }
         __syncthreads();
         dec_copy_out<THREADS>(buf, s_img, O, tile_syms, 0u, total);
     } else {
