@@ -118,6 +118,16 @@ def expected_symbols(n, gap_elements, max_symbols):
     return np.minimum(np.asarray(max_symbols, np.int64), (5 * n + 4 * g - 1) // (4 * g) + 2)
 
 
+# Decode launches are split at these expected codewords per segment: one launch sizes its LDS
+# image for its densest member, so sparse buffers go in launches of their own and keep the
+# occupancy their density allows (at most 32, 33-64 and more than 64).
+DENSITY_CLASSES = (32, 64)
+
+
+def density_class(expected):
+    return np.searchsorted(DENSITY_CLASSES, np.asarray(expected), side="left")
+
+
 class BatchCodec:
     """Encodes and decodes a list of device uint8 buffers in batched launches."""
 
@@ -221,9 +231,11 @@ class BatchCodec:
         self.dec_launches = []
         ms = iv["max_symbols_per_segment"].astype(np.int64)
         tw = iv["bytes"].astype(np.int64) // 4  # words of each table image
-        # one launch per decoder shape; its LDS image is sized from the members' largest table
-        # and densest segments (tiles that outgrow it go through the region in windows)
-        dkeys = np.stack([d["dec_threads"], d["lookback_window"]], 1)
+        # one launch per decoder shape and density class; its LDS image is sized from the
+        # members' largest table and densest segments (tiles that outgrow it go through the
+        # region in windows)
+        expect = expected_symbols(self.ns, self.gap_elements, ms)
+        dkeys = np.stack([d["dec_threads"], d["lookback_window"], density_class(expect)], 1)
         for key in np.unique(dkeys[live], axis=0):
             idx = live[(dkeys[live] == key).all(1)]
             cfg = _cfg_struct(dict(dec_threads=key[0], lookback_window=key[1]))
@@ -239,7 +251,7 @@ class BatchCodec:
             jobs["l1_entries"] = iv["l1_entries"][idx]
             jobs["table_words"] = tw[idx]
             jobs["max_symbols"] = ms[idx]
-            dense = expected_symbols(self.ns[idx], self.gap_elements[idx], ms[idx])
+            dense = expect[idx]
             jobs["max_length"] = iv["max_length"][idx]
             total = lib.hgap_batch_plan_decode(_vp(jobs), len(idx), ctypes.byref(cfg))
             self.dec_launches.append(dict(cfg=cfg, jobs=_to_device(jobs, self.device),

@@ -94,6 +94,12 @@ def histogram(d_in, stream=None):
     return hist
 
 
+RANGE_BYTES = 256 * 1024  # HGAP_RANGE_BYTES
+# HGAP_CHAINED_MIN_BYTES: below this the chained encoder has too few ranges to fill the device,
+# and hgap_encode_ranges without a launch shape takes the per-tile look-back encoder
+CHAINED_MIN_BYTES = 64 << 20
+
+
 def histogram_ranges(d_in, stream=None, range_hist=None):
     """(hist, range_hist): the 256 byte counts, and the counts of every range of
     HGAP_RANGE_BYTES input bytes (int32 tensor of range_count(n) x 256) for encode()."""
@@ -199,7 +205,10 @@ def encode(d_in, cb=None, hist=None, sizes=None, stream=None, workspace=None, pa
     n = d_in.numel()
     if cb is None or (sizes is None and hist is None):
         if hist is None:
-            hist, range_hist = histogram_ranges(d_in, stream)
+            if n >= CHAINED_MIN_BYTES:
+                hist, range_hist = histogram_ranges(d_in, stream)
+            else:
+                hist = histogram(d_in, stream)
         hist = _hist_array(hist)
         if cb is None:
             cb = codebook_build(hist)

@@ -1292,13 +1292,25 @@ struct DecJobDev {
 // takes the contiguous tiles [b*C, (b+1)*C) (C = ceil(total / grid)), so consecutive tiles of a
 // workgroup mostly share a job and its table is loaded once per job.  The offsets kernel scans
 // all tiles; a tile's offset inside its job is its prefix minus that of the job's first tile.
+// a job descriptor read into scalar registers (every lane reads the same one)
+__device__ __forceinline__ DecJobDev dec_load_job(const DecJobDev *p) {
+    static_assert(sizeof(DecJobDev) % 4 == 0, "DecJobDev is whole words");
+    DecJobDev job;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(p);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&job);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(DecJobDev) / 4; i++)
+        dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return job;
+}
+
 struct DecCursor {
     const DecJobDev *jobs;
     uint32_t count, j;
     DecJobDev job;
     // the job owning tile g (g at or after the current job's first tile)
     __device__ __forceinline__ void seek(uint32_t g) {
-        while (g >= job.base + job.num_tiles && j + 1 < count) job = jobs[++j];
+        while (g >= job.base + job.num_tiles && j + 1 < count) job = dec_load_job(&jobs[++j]);
     }
 };
 
@@ -1316,12 +1328,57 @@ __device__ __forceinline__ DecBuffer dec_job_buffer(const DecJobDev &job) {
     return buf;
 }
 
-// loads job `j`'s table into LDS (after a barrier: the previous table may still be in use)
+// whether a job's decode table has one level (no second-level entries after the first level)
+__device__ __forceinline__ bool dec_job_single(const DecJobDev &job) {
+    return job.table_words == job.l1_entries + kDecLengthWords;
+}
+
+// loads job `j`'s table into LDS (after a barrier: the previous table may still be in use);
+// `rotate`: one-level entries rotated right by 8 for the count walk (EntRotated)
 template <int THREADS>
-__device__ __forceinline__ void dec_load_job_table(uint32_t *s_tab, const DecJobDev &job) {
+__device__ __forceinline__ void dec_load_job_table(uint32_t *s_tab, const DecJobDev &job,
+                                                   bool rotate = false) {
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < job.table_words; i += THREADS) s_tab[i] = job.table[i];
+    const uint32_t entries = job.table_words - kDecLengthWords;
+    for (uint32_t i = threadIdx.x; i < job.table_words; i += THREADS) {
+        const uint32_t e = job.table[i];
+        s_tab[i] = rotate && i < entries ? __builtin_amdgcn_alignbit(e, e, 8) : e;
+    }
     __syncthreads();
+}
+
+// A tile's inputs in the batched kernels: loaded one tile ahead while the next tile of the
+// workgroup's run belongs to the same job (the usual case), else at the tile's start.
+struct DecBatchStage {
+    DecStage st;
+    uint32_t cnt;  // the write kernel's segment count (0 in the count kernel)
+    bool ready;    // loaded ahead (uniform)
+};
+
+template <int THREADS, bool WITH_COUNT>
+__device__ __forceinline__ DecBatchStage dec_batch_fetch(const DecJobDev &job, uint32_t g,
+                                                         const uint8_t *seg_counts) {
+    DecBatchStage s;
+    s.st = dec_fetch<THREADS>(dec_job_buffer(job), g - job.base);
+    s.cnt = WITH_COUNT ? seg_counts[(uint64_t)g * THREADS + threadIdx.x] : 0u;
+    s.ready = true;
+    return s;
+}
+
+// this tile's inputs (`ahead` if it was loaded ahead), then starts loading the next tile's when
+// it is in the same job
+template <int THREADS, bool WITH_COUNT>
+__device__ __forceinline__ DecBatchStage dec_batch_stage(DecBatchStage &ahead,
+                                                         const DecJobDev &job, uint32_t g,
+                                                         uint32_t g1,
+                                                         const uint8_t *seg_counts) {
+    const DecBatchStage now =
+        ahead.ready ? ahead : dec_batch_fetch<THREADS, WITH_COUNT>(job, g, seg_counts);
+    if (g + 1 < g1 && g + 1 < job.base + job.num_tiles)
+        ahead = dec_batch_fetch<THREADS, WITH_COUNT>(job, g + 1, seg_counts);
+    else
+        ahead.ready = false;
+    return now;
 }
 
 template <int THREADS>
@@ -1335,26 +1392,30 @@ void dec_count_batch_kernel(const DecJobDev *__restrict__ jobs, uint32_t count, 
     const uint32_t g1 = g0 + chunk < total ? g0 + chunk : total;
     if (g0 >= g1) return;  // uniform
     DecCursor cur{jobs, count, find_job(jobs, count, g0), {}};
-    cur.job = jobs[cur.j];
+    cur.job = dec_load_job(&jobs[cur.j]);
+    DecBatchStage ahead{};
     uint32_t loaded = 0xFFFFFFFFu;
+    bool single = false;
     for (uint32_t g = g0; g < g1; g++) {
         cur.seek(g);
         if (cur.j != loaded) {  // uniform
-            dec_load_job_table<THREADS>(s_tab, cur.job);
+            single = dec_job_single(cur.job);
+            dec_load_job_table<THREADS>(s_tab, cur.job, single);
             loaded = cur.j;
         }
+        const DecStage st =
+            dec_batch_stage<THREADS, false>(ahead, cur.job, g, g1, nullptr).st;
         const DecBuffer buf = dec_job_buffer(cur.job);
         const uint32_t tile = g - cur.job.base;
-        const DecStage st = dec_fetch<THREADS>(buf, tile);
         const uint32_t seg = tile * THREADS + threadIdx.x;
         const DecSeg sg = dec_segment(buf, st, seg);
         const uint8_t *s_len =
             reinterpret_cast<const uint8_t *>(s_tab + cur.job.table_words - kDecLengthWords);
-        const uint32_t cnt =
-            seg < buf.gap_elements
-                ? dec_count<false, EntStored>(sg, s_tab, s_len, cur.job.prefix_bits,
-                                              cur.job.l1_entries)
-                : 0u;
+        const uint32_t P = cur.job.prefix_bits, l1 = cur.job.l1_entries;
+        uint32_t cnt = 0u;
+        if (seg < buf.gap_elements)
+            cnt = single ? dec_count<true, EntRotated>(sg, s_tab, s_len, P, l1)
+                         : dec_count<false, EntStored>(sg, s_tab, s_len, P, l1);
         seg_counts[(uint64_t)g * THREADS + threadIdx.x] = (uint8_t)cnt;
         const uint32_t wsum = wave_incl_sum(cnt);
         if (lane_id() == 63) wave_sums[(uint64_t)g * (THREADS / 64) + (threadIdx.x >> 6)] = wsum;
@@ -1376,22 +1437,29 @@ void dec_write_batch_kernel(const DecJobDev *__restrict__ jobs, uint32_t count, 
     const uint32_t g1 = g0 + chunk < total ? g0 + chunk : total;
     if (g0 >= g1) return;  // uniform
     DecCursor cur{jobs, count, find_job(jobs, count, g0), {}};
-    cur.job = jobs[cur.j];
+    cur.job = dec_load_job(&jobs[cur.j]);
+    DecBatchStage ahead{};
     uint32_t loaded = 0xFFFFFFFFu;
+    bool single = false;
     for (uint32_t g = g0; g < g1; g++) {
         cur.seek(g);
         if (cur.j != loaded) {  // uniform; its barriers also end the previous tile's copy-out
+            single = dec_job_single(cur.job);
             dec_load_job_table<THREADS>(s_tab, cur.job);
             loaded = cur.j;
         }
+        const DecBatchStage now = dec_batch_stage<THREADS, true>(ahead, cur.job, g, g1, seg_counts);
         const DecBuffer buf = dec_job_buffer(cur.job);
         const uint32_t tile = g - cur.job.base;
-        const DecStage st = dec_fetch<THREADS>(buf, tile);
-        const uint32_t cnt = seg_counts[(uint64_t)g * THREADS + threadIdx.x];
         const uint64_t O = tile_offsets[g] - tile_offsets[cur.job.base];
         const DecTable t{cur.job.prefix_bits, cur.job.l1_entries};
-        dec_write_tile<THREADS, false>(buf, t, s_tab, s_img, image_bytes, s_wave, st,
-                                       tile * THREADS + threadIdx.x, cnt, O);
+        const uint32_t seg = tile * THREADS + threadIdx.x;
+        if (single)
+            dec_write_tile<THREADS, true>(buf, t, s_tab, s_img, image_bytes, s_wave, now.st, seg,
+                                          now.cnt, O);
+        else
+            dec_write_tile<THREADS, false>(buf, t, s_tab, s_img, image_bytes, s_wave, now.st,
+                                           seg, now.cnt, O);
     }
 }
 
@@ -1977,10 +2045,13 @@ hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook
                                uint32_t *d_payload, uint32_t *d_gap, void *d_workspace,
                                size_t workspace_bytes, const hgap_launch_config *cfg,
                                hgap_stream_t stream) {
-    if (n > 0 && !d_range_hist) return HGAP_ERR_INVALID_ARG;
+    if (n > 0 && (!d_range_hist || !aligned16(d_range_hist))) return HGAP_ERR_INVALID_ARG;
     if (cfg)
         return encode_common(d_in, n, cb, sizes, d_range_hist, d_payload, d_gap, d_workspace,
                              workspace_bytes, cfg, stream);
+    if (n < HGAP_CHAINED_MIN_BYTES)
+        return hgap_encode(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
+                           stream);
     hgap_launch_config c;
     hgap_default_config(&c);
     hgap_status st = HGAP_ERR_UNSUPPORTED;

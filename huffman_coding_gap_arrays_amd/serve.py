@@ -168,7 +168,10 @@ class _Batch:
             table = s.tables.get(r.compressed.codebook, int(cfg.prefix_bits))
             out = torch.empty(_up(16, r.n + 16), dtype=torch.uint8, device=s.device)
             r.result = out
-            key = (cfg.dec_threads, cfg.lookback_window)
+            e = r.compressed
+            expect = batch.expected_symbols(r.n, e.gap_elements,
+                                            table.info.max_symbols_per_segment)
+            key = (cfg.dec_threads, cfg.lookback_window, int(batch.density_class(expect)))
             groups.setdefault(key, (cfg, []))[1].append((r, table, out))
             self.keep.append(table)
         for cfg, members in groups.values():

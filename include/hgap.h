@@ -157,8 +157,11 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
  * hgap_histogram_ranges over the same input (other counts give a wrong stream, never an access
  * outside the caller's buffers), derives every range's bit offset from them before encoding,
  * and so needs no tile to wait for another.  Workspace: hgap_encode_workspace_bytes(n).
- * cfg NULL = the default shape, falling back to smaller tiles like hgap_encode. */
+ * cfg NULL = the default shape, falling back to smaller tiles like hgap_encode; inputs under
+ * HGAP_CHAINED_MIN_BYTES then take hgap_encode's per-tile look-back instead (too few ranges
+ * to fill the device; the counts go unused).  A non-NULL cfg always runs the chained encoder. */
 #define HGAP_RANGE_BYTES (256u * 1024u)
+#define HGAP_CHAINED_MIN_BYTES (64ull << 20)
 uint32_t hgap_range_count(uint64_t n);
 hgap_status hgap_histogram_ranges(const void *d_in, uint64_t n, uint32_t *d_hist,
                                   uint32_t *d_range_hist, hgap_stream_t stream);

@@ -84,6 +84,26 @@ def test_batched_encode_decode_bit_exact(batch_data, orc, mixed):
     assert bc.mismatches() == 0
 
 
+def test_batched_decode_table_kinds_and_density_classes(batch_data):
+    """The batch mixes one-level and two-level decode tables (the kernels pick the walk per
+    tile) and spans several density classes (one decode launch each); all decode exactly."""
+    import torch
+
+    batch, codec, arena, bufs = batch_data
+    bc = batch.BatchCodec(bufs)
+    live = bc.ns > 0
+    l2 = bc.dec_info["l2_entries"][live]
+    assert (l2 == 0).any() and (l2 > 0).any()
+    expect = batch.expected_symbols(bc.ns, bc.gap_elements,
+                                    bc.dec_info["max_symbols_per_segment"])
+    classes = set(batch.density_class(expect[live]).tolist())
+    assert len(classes) >= 2
+    assert len(bc.dec_launches) == len(classes)
+    bc.run()
+    torch.cuda.synchronize()
+    assert bc.mismatches() == 0
+
+
 def test_batched_decode_through_small_image(batch_data, monkeypatch):
     """A batched decode whose LDS image is sized for one codeword per segment sends every tile
     through the image in windows; every buffer still decodes to its input."""

@@ -111,9 +111,13 @@ def test_histogram_ranges_and_chained_encode(codec, orc, dist):
         hist = orc.histogram(d)
         ocb = orc.codebook(hist)
         pay, gap = orc.encode(d, ocb)
-        enc = codec.encode(d_dev, hist=hist, range_hist=rh)
+        # an explicit (default) shape runs the chained encoder at any size
+        enc = codec.encode(d_dev, hist=hist, range_hist=rh, config={})
         assert (enc.payload_u32() == pay).all(), f"{dist} n={n}: payload"
         assert (enc.gap_u32() == gap).all(), f"{dist} n={n}: gap"
+        # without one, small inputs take the per-tile look-back: the same stream
+        enc2 = codec.encode(d_dev, hist=hist, range_hist=rh)
+        assert (enc2.payload_u32() == pay).all() and (enc2.gap_u32() == gap).all()
         table = codec.DeviceTable(enc.codebook)
         out = codec.decode(enc.payload, enc.gap, enc.payload_words, enc.gap_elements, table, n)
         assert (out.cpu().numpy() == d).all(), f"{dist} n={n}: round trip"
@@ -305,7 +309,7 @@ def test_full_size_configs(codec, orc, golden_codebooks, name):
     h2, rh = codec.histogram_ranges(d_dev)
     assert (codec.host_histogram(h2).astype(np.uint64) == h).all()
     W, GW = enc.payload_words, int(enc.sizes.gap_words)
-    enc2 = codec.encode(d_dev, cb=enc.codebook, sizes=enc.sizes, range_hist=rh)
+    enc2 = codec.encode(d_dev, cb=enc.codebook, sizes=enc.sizes, range_hist=rh, config={})
     assert torch.equal(enc2.payload[:W], enc.payload[:W]), "chained payload"
     assert torch.equal(enc2.gap[:GW], enc.gap[:GW]), "chained gap"
     del enc2, rh
