@@ -102,7 +102,8 @@ class HistJob(ctypes.Structure):
     """hgap_hist_job (one buffer of a batched histogram)."""
 
     _fields_ = [("in_", ctypes.c_uint64), ("n", ctypes.c_uint64), ("hist", ctypes.c_uint64),
-                ("reserved", ctypes.c_uint32 * 2), ("base", ctypes.c_uint64)]
+                ("range_base", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("base", ctypes.c_uint64)]
 
 
 class EncJob(ctypes.Structure):
@@ -112,7 +113,8 @@ class EncJob(ctypes.Structure):
                 ("gap", ctypes.c_uint64), ("payload_words", ctypes.c_uint32),
                 ("gap_words", ctypes.c_uint32), ("table", ctypes.c_uint32),
                 ("max_len", ctypes.c_uint32), ("base", ctypes.c_uint32),
-                ("num_tiles", ctypes.c_uint32)]
+                ("num_tiles", ctypes.c_uint32), ("range_base", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class DecJob(ctypes.Structure):
@@ -222,6 +224,11 @@ PROTOTYPES = {
     "hgap_batch_workspace_bytes": (_SZ, [_U32]),
     "hgap_batch_decode_workspace_bytes": (_SZ, [_U32, ctypes.POINTER(LaunchConfig)]),
     "hgap_batch_encode": (_INT, [_P, _U32, _U32, _U32, _P, _P, _SZ, ctypes.c_void_p, _P]),
+    "hgap_batch_histogram_ranges": (_INT, [_P, _U32, _U32, _P, _P]),
+    "hgap_batch_plan_encode_ranges": (_U32, [_P, _U32, ctypes.c_void_p]),
+    "hgap_batch_encode_ranges_workspace_bytes": (_SZ, [_U32]),
+    "hgap_batch_encode_ranges": (_INT, [_P, _U32, _U32, _U32, _P, _P, _P, _SZ, ctypes.c_void_p,
+                                        _P]),
     "hgap_batch_decode": (_INT, [_P, _U32, _U32, _U32, _U32, _P, _SZ, ctypes.c_void_p, _P]),
     "hgap_synth_fill": (_INT, [_P, _U64, _U64, _U64, _P, _P, _P]),
     "hgap_batch_prepare": (_INT, [_P, _U32, _P, _P, _P, _U32]),

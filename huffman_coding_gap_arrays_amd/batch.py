@@ -86,6 +86,28 @@ def histograms(buffers, stream=None, device="cuda"):
     return hist[:B].cpu().numpy().view(np.uint32)
 
 
+def histograms_ranges(buffers, stream=None, device="cuda"):
+    """(hists, range_hist, range_base): the 256 counts per buffer (uint32 [B, 256] on the host)
+    and the counts of every 256 KiB range (int32 device tensor, one row per range; buffer i's
+    rows start at range_base[i]), in one batched launch."""
+    B = len(buffers)
+    hist = torch.empty((max(B, 1), 256), dtype=torch.int32, device=device)
+    jobs = np.zeros(B, HIST_JOB)
+    jobs["in_"] = [b.data_ptr() for b in buffers]
+    jobs["n"] = [b.numel() for b in buffers]
+    jobs["hist"] = hist.data_ptr() + 1024 * np.arange(B, dtype=np.uint64)
+    _lib().hgap_batch_plan_histogram(_vp(jobs), B)
+    ranges = -(-jobs["n"].astype(np.int64) // codec.RANGE_BYTES)
+    total = int(ranges.sum())
+    range_hist = torch.empty((max(total, 1), 256), dtype=torch.int32, device=device)
+    d_jobs = _to_device(jobs, device)
+    check(_lib().hgap_batch_histogram_ranges(ctypes.c_void_p(d_jobs.data_ptr()), B, total,
+                                             ctypes.c_void_p(range_hist.data_ptr()),
+                                             _stream(stream)), "hgap_batch_histogram_ranges")
+    return (hist[:B].cpu().numpy().view(np.uint32), range_hist,
+            jobs["range_base"].astype(np.uint32).copy())
+
+
 def codebooks(hists, threads=0):
     """Codebooks of [B, 256] histograms, built on a pool of host threads."""
     h = np.ascontiguousarray(hists, dtype=np.uint32)
@@ -131,20 +153,32 @@ def density_class(expected):
 class BatchCodec:
     """Encodes and decodes a list of device uint8 buffers in batched launches."""
 
+    # an encode launch with at least this many 256 KiB ranges takes the chained encoder
+    CHAINED_MIN_RANGES = 512
+
     def __init__(self, buffers, configs=None, hists=None, device="cuda", stream=None,
-                 threads=0, dec_configs=None, predictor=None):
+                 threads=0, dec_configs=None, predictor=None, chained=None):
         """configs[i] / dec_configs[i]: launch shapes of buffer i's encode / decode (dicts of
         hgap_launch_config fields; dec_configs defaults to configs).  With a `predictor`
         (predictor.Predictor), both come from its per-buffer choice instead.  Host preparation
-        (codebooks, sizes, code tables, decode tables) runs on `threads` host threads."""
+        (codebooks, sizes, code tables, decode tables) runs on `threads` host threads.
+        Without `hists`, the batched histogram also counts every 256 KiB range, and encode
+        launches use the chained encoder: when they have CHAINED_MIN_RANGES ranges or more
+        (chained=None), always (True) or never (False)."""
         from . import predictor as pr
 
         lib = _lib()
         self.device = device
         self.buffers = list(buffers)
         B = self.count = len(self.buffers)
+        self.range_hist = self.range_base = None
+        self.chained = chained
         if hists is None:
-            hists = histograms(self.buffers, stream, device)
+            if chained is False:
+                hists = histograms(self.buffers, stream, device)
+            else:
+                hists, self.range_hist, self.range_base = histograms_ranges(self.buffers, stream,
+                                                                            device)
         self.hists = np.ascontiguousarray(hists, dtype=np.uint32).reshape(B, 256)
         self.cb_array = (Codebook * max(B, 1))()
         self.size_array = (Sizes * max(B, 1))()
@@ -223,11 +257,22 @@ class BatchCodec:
             jobs["gap_words"] = self.gap_words[idx]
             jobs["table"] = idx
             jobs["max_len"] = self.max_len[idx]
-            total = lib.hgap_batch_plan_encode(_vp(jobs), len(idx), ctypes.byref(cfg))
+            chained = False
+            if self.range_hist is not None and self.chained is not False:
+                ranges = int((-(-self.ns[idx].astype(np.int64) // codec.RANGE_BYTES)).sum())
+                chained = self.chained or ranges >= self.CHAINED_MIN_RANGES
+            if chained:
+                jobs["range_base"] = self.range_base[idx]
+                total = lib.hgap_batch_plan_encode_ranges(_vp(jobs), len(idx), ctypes.byref(cfg))
+                ws = torch.empty(_up(16, lib.hgap_batch_encode_ranges_workspace_bytes(total)),
+                                 dtype=torch.uint8, device=self.device)
+            else:
+                total = lib.hgap_batch_plan_encode(_vp(jobs), len(idx), ctypes.byref(cfg))
+                ws = self._workspace(total)
             self.enc_launches.append(dict(cfg=cfg, jobs=_to_device(jobs, self.device),
-                                          count=len(idx), total=total,
+                                          count=len(idx), total=total, chained=chained,
                                           max_len=int(self.max_len[idx].max()),
-                                          ws=self._workspace(total), members=idx))
+                                          ws=ws, members=idx))
         self.dec_launches = []
         ms = iv["max_symbols_per_segment"].astype(np.int64)
         tw = iv["bytes"].astype(np.int64) // 4  # words of each table image
@@ -263,6 +308,14 @@ class BatchCodec:
     def encode(self, stream=None):
         lib = _lib()
         for L in self.enc_launches:
+            if L["chained"]:
+                check(lib.hgap_batch_encode_ranges(
+                    ctypes.c_void_p(L["jobs"].data_ptr()), L["count"], L["total"], L["max_len"],
+                    ctypes.c_void_p(self.d_enc_tables.data_ptr()),
+                    ctypes.c_void_p(self.range_hist.data_ptr()),
+                    ctypes.c_void_p(L["ws"].data_ptr()), L["ws"].numel(),
+                    ctypes.byref(L["cfg"]), _stream(stream)), "hgap_batch_encode_ranges")
+                continue
             check(lib.hgap_batch_encode(ctypes.c_void_p(L["jobs"].data_ptr()), L["count"],
                                         L["total"], L["max_len"],
                                         ctypes.c_void_p(self.d_enc_tables.data_ptr()),

@@ -246,6 +246,47 @@ __global__ __launch_bounds__(kHistThreadsSingle) void hist_kernel(const uint8_t 
 // counts whole ranges (ranges blockIdx.x, blockIdx.x + gridDim.x, ...); after each one,
 // thread b < 256 sums bin b over the 64 copies, clears them as it reads, and stores the range's
 // count; it keeps a running total of bin b for the one global atomic at the end.
+// Counts bytes [lo, hi) of `in` (lo a multiple of 16) into the lane copies `mine`: 16-byte
+// loads, four in flight per thread, then the last hi % 16 bytes one per thread.
+template <int THREADS>
+__device__ __forceinline__ void hist_count_range(const uint8_t *in, uint64_t lo, uint64_t hi,
+                                                 uint32_t *mine) {
+    const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
+    const uint64_t v1 = hi >> 4;
+    uint64_t v = (lo >> 4) + threadIdx.x;
+    for (; v + 3 * THREADS < v1; v += 4 * THREADS) {
+        u32x4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = __builtin_nontemporal_load(&in4[v + u * THREADS]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
+        }
+    }
+    for (; v < v1; v += THREADS) {
+        const u32x4 q = __builtin_nontemporal_load(&in4[v]);
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
+    }
+    const uint64_t t = (v1 << 4) + threadIdx.x;
+    if (t < hi) atomicAdd(&mine[(uint32_t)in[t] << 6], 1u);
+}
+
+// After a barrier, thread b < 256: the sum of bin b over the 64 copies, which it clears (read
+// in a rotated order so that the lanes of a pass hit different banks).
+__device__ __forceinline__ uint32_t hist_take_bin(uint32_t *s_bins) {
+    uint32_t sum = 0;
+    for (int c = 0; c < kHistCopies; c++) {
+        const uint32_t i = (threadIdx.x << 6) + ((c + threadIdx.x) & 63u);
+        sum += s_bins[i];
+        s_bins[i] = 0u;
+    }
+    return sum;
+}
+
 __global__ __launch_bounds__(kHistThreadsSingle) void hist_range_kernel(
     const uint8_t *__restrict__ in, uint64_t n, uint32_t *__restrict__ hist,
     uint32_t *__restrict__ range_hist, uint32_t ranges, uint32_t range_bytes) {
@@ -254,43 +295,13 @@ __global__ __launch_bounds__(kHistThreadsSingle) void hist_range_kernel(
     for (int i = threadIdx.x; i < kHistCopies * 256; i += kThreads) s_bins[i] = 0u;
     __syncthreads();
     uint32_t *mine = s_bins + lane_id();
-    const u32x4 *in4 = reinterpret_cast<const u32x4 *>(in);
     uint32_t total = 0;
     for (uint32_t r = blockIdx.x; r < ranges; r += gridDim.x) {
         const uint64_t lo = (uint64_t)r * range_bytes;
-        const uint64_t hi = lo + range_bytes < n ? lo + range_bytes : n;
-        const uint64_t v1 = hi >> 4;  // lo is a multiple of 16
-        uint64_t v = (lo >> 4) + threadIdx.x;
-        for (; v + 3 * kThreads < v1; v += 4 * kThreads) {
-            u32x4 q[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) q[u] = __builtin_nontemporal_load(&in4[v + u * kThreads]);
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-#pragma unroll
-                for (int j = 0; j < 16; j++)
-                    atomicAdd(&mine[((q[u][j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
-            }
-        }
-        for (; v < v1; v += kThreads) {
-            const u32x4 q = __builtin_nontemporal_load(&in4[v]);
-#pragma unroll
-            for (int j = 0; j < 16; j++)
-                atomicAdd(&mine[((q[j >> 2] >> (8 * (j & 3))) & 0xFFu) << 6], 1u);
-        }
-        // tail bytes (n % 16) of the last range
-        if (hi == n) {
-            const uint64_t t = (v1 << 4) + threadIdx.x;
-            if (t < n) atomicAdd(&mine[(uint32_t)in[t] << 6], 1u);
-        }
+        hist_count_range<kThreads>(in, lo, lo + range_bytes < n ? lo + range_bytes : n, mine);
         __syncthreads();
         if (threadIdx.x < 256) {
-            uint32_t sum = 0;
-            for (int c = 0; c < kHistCopies; c++) {
-                const uint32_t i = (threadIdx.x << 6) + ((c + threadIdx.x) & 63u);
-                sum += s_bins[i];
-                s_bins[i] = 0u;
-            }
+            const uint32_t sum = hist_take_bin(s_bins);
             range_hist[256ull * r + threadIdx.x] = sum;
             total += sum;
         }
@@ -1142,7 +1153,8 @@ struct HistJobDev {
     const uint8_t *in;
     uint64_t n;
     uint32_t *hist;
-    uint32_t reserved[2];
+    uint32_t range_base;  // first row of this job's per-range counts
+    uint32_t reserved;
     uint64_t base;  // first byte of this job in the batch's concatenation
 };
 
@@ -1201,6 +1213,45 @@ __global__ __launch_bounds__(kHistThreads) void hist_batch_kernel(const HistJobD
     }
 }
 
+// Per-range counts of a batch (hist_range_kernel over many buffers): workgroup b takes the
+// contiguous ranges [b*C, (b+1)*C) of the batch (range_base order) and, after each one, stores
+// its counts and adds them to the job's histogram (the job's histogram words were zeroed).
+__global__ __launch_bounds__(kHistThreadsSingle) void hist_range_batch_kernel(
+    const HistJobDev *__restrict__ jobs, uint32_t count, uint32_t total_ranges,
+    uint32_t range_bytes, uint32_t *__restrict__ range_hist) {
+    constexpr int kThreads = kHistThreadsSingle;
+    __shared__ uint32_t s_bins[kHistCopies * 256];
+    for (int i = threadIdx.x; i < kHistCopies * 256; i += kThreads) s_bins[i] = 0u;
+    __syncthreads();
+    uint32_t *mine = s_bins + lane_id();
+    const uint32_t chunk = (total_ranges + gridDim.x - 1) / gridDim.x;
+    const uint32_t u0 = blockIdx.x * chunk;
+    const uint32_t u1 = u0 + chunk < total_ranges ? u0 + chunk : total_ranges;
+    if (u0 >= u1) return;  // uniform
+    // the last job whose first range is at or before u0 (jobs without bytes own no ranges)
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].range_base <= u0) lo = mid; else hi = mid - 1;
+    }
+    uint32_t j = lo;
+    for (uint32_t u = u0; u < u1; u++) {
+        while (j + 1 < count && jobs[j + 1].range_base <= u) j++;
+        const HistJobDev job = jobs[j];
+        const uint64_t a = (uint64_t)(u - job.range_base) * range_bytes;
+        if (a < job.n)  // always, when total_ranges matches the jobs
+            hist_count_range<kThreads>(job.in, a,
+                                       a + range_bytes < job.n ? a + range_bytes : job.n, mine);
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            const uint32_t sum = hist_take_bin(s_bins);
+            range_hist[256ull * u + threadIdx.x] = sum;
+            if (sum) atomicAdd(&job.hist[threadIdx.x], sum);
+        }
+        __syncthreads();
+    }
+}
+
 struct EncJobDev {
     const uint8_t *in;
     uint64_t n;
@@ -1210,8 +1261,10 @@ struct EncJobDev {
     uint32_t gap_words;
     uint32_t table;    // index of this job's 256-word code table
     uint32_t max_len;
-    uint32_t base;     // first tile (and look-back status word) of this job
+    uint32_t base;     // first tile (chained: first range) and look-back status word of this job
     uint32_t num_tiles;
+    uint32_t range_base;  // chained: first row of this job's per-range counts
+    uint32_t reserved;
 };
 
 // zero every job's gap words (one workgroup per job, strided)
@@ -1267,6 +1320,100 @@ __global__ __launch_bounds__(THREADS) void enc_batch_kernel(
         EncChain unused{};
         enc_tile<THREADS, K, false>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl,
                                     wpl, none, unused);
+    }
+}
+
+// Chained batched encode (the chained encoder for every job).  Ranges of all jobs are numbered
+// in job order (job.base = the job's first); a job's status words are [base, base + ranges).
+// The prep kernel stores every range's bit count (its row of range counts dotted with the
+// job's code lengths) as a look-back aggregate, a job's first range as an inclusive prefix.
+// One wave per range; it also records the range's job (range_job[u]), so that the encoder
+// needs no search per range.
+__global__ __launch_bounds__(256) void enc_range_prep_batch_kernel(
+    const EncJobDev *__restrict__ jobs, uint32_t count, uint32_t total_ranges,
+    const uint32_t *__restrict__ tables, const uint32_t *__restrict__ range_hist,
+    uint32_t range_bytes, uint64_t *__restrict__ status, uint32_t *__restrict__ range_job,
+    uint32_t *__restrict__ counter) {
+    const uint32_t lane = lane_id();
+    const uint32_t waves = gridDim.x * 4u;
+    for (uint32_t u = blockIdx.x * 4u + (threadIdx.x >> 6); u < total_ranges; u += waves) {
+        const uint32_t j = find_job(jobs, count, u);
+        const EncJobDev job = jobs[j];
+        const uint32_t r = u - job.base;
+        if (lane == 0) range_job[u] = j;
+        if ((uint64_t)r * range_bytes >= job.n) {  // only when the plan does not match
+            if (lane == 0) status_store(&status[u], kFlagA);
+            continue;
+        }
+        const uint32_t *len = tables + 256ull * job.table + 4 * lane;
+        const u32x4 h = *reinterpret_cast<const u32x4 *>(
+            range_hist + 256ull * (job.range_base + r) + 4 * lane);
+        const uint64_t bits = wave_sum_u64(
+            (uint64_t)h[0] * (len[0] & 31u) + (uint64_t)h[1] * (len[1] & 31u) +
+            (uint64_t)h[2] * (len[2] & 31u) + (uint64_t)h[3] * (len[3] & 31u));
+        if (lane == 0) status_store(&status[u], (r == 0 ? kFlagP : kFlagA) | (bits & kValueMask));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0u;
+}
+
+// A workgroup takes a range from the counter, reloads the code table when the job changes, and
+// encodes the range's tiles in order (enc_tile's chained mode; the range's first tile claims
+// the workgroup's next range).
+template <int THREADS, int K>
+__global__ __launch_bounds__(THREADS) void enc_range_batch_kernel(
+    const EncJobDev *__restrict__ jobs, uint32_t count, uint32_t total_ranges,
+    const uint32_t *__restrict__ tables, uint32_t *__restrict__ counter,
+    uint64_t *__restrict__ status, const uint32_t *__restrict__ range_job, uint32_t out_words,
+    uint32_t wpl, uint32_t tiles_per_range) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
+    __shared__ uint32_t s_tab[256];
+    __shared__ uint32_t s_wave[THREADS / 64];
+    __shared__ uint64_t s_excl;
+    __shared__ uint32_t s_ticket;
+    uint32_t *s_out = s_dyn;
+    uint32_t *s_gap = s_dyn + out_words;
+    if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1u);
+    __syncthreads();
+    EncChain ch{};
+    ch.next_range = s_ticket;
+    ch.tiles_per_range = tiles_per_range;
+    NextTile next{counter, &s_ticket, 0u, nullptr};
+    uint32_t loaded = 0xFFFFFFFFu;
+    while (ch.next_range < total_ranges) {  // uniform
+        const uint32_t j = range_job[ch.next_range];
+        const EncJobDev job = jobs[j < count ? j : count - 1];
+        if (job.table != loaded) {  // the previous tile ended with a barrier
+            for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = tables[256u * job.table + i];
+            loaded = job.table;
+            __syncthreads();
+        }
+        EncBuffer buf;
+        buf.in = job.in;
+        buf.n = job.n;
+        buf.payload = job.payload;
+        buf.payload_words = job.payload_words;
+        buf.gap_words = job.gap_words;
+        buf.gap = job.gap;
+        buf.status = status + job.base;
+        buf.num_tiles = job.num_tiles;
+        buf.reserved = 0;
+        ch.range = ch.next_range - job.base;
+        ch.ranges = (job.num_tiles + tiles_per_range - 1) / tiles_per_range;
+        const uint32_t lo = ch.range * tiles_per_range;
+        if (lo >= job.num_tiles) {  // a range without tiles (only when the plan does not match)
+            __syncthreads();
+            if (threadIdx.x == 0) s_ticket = atomicAdd(counter, 1u);
+            __syncthreads();
+            ch.next_range = s_ticket;
+            continue;
+        }
+        ch.end = lo + tiles_per_range < job.num_tiles ? lo + tiles_per_range : job.num_tiles;
+        for (uint32_t t = lo; t < ch.end; t++) {
+            ch.first = t == lo;
+            const EncStage<K> st = enc_fetch<THREADS, K>(buf, t);
+            enc_tile<THREADS, K, true>(buf, t, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
+                                       next, ch);
+        }
     }
 }
 
@@ -1811,6 +1958,38 @@ static hgap_status launch_enc_batch(const EncJobDev *jobs, uint32_t count, uint3
     return HGAP_ERR_UNSUPPORTED;
 }
 
+template <int THREADS, int K>
+static hgap_status launch_enc_range_batch_t(const EncJobDev *jobs, uint32_t count,
+                                            uint32_t total_ranges, const uint32_t *tables,
+                                            uint32_t *ws, uint32_t max_len,
+                                            const hgap_launch_config &c, hipStream_t s) {
+    constexpr uint32_t tile = THREADS * K;
+    const uint32_t out_words = enc_out_words(tile, max_len);
+    const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
+    auto kern = enc_range_batch_kernel<THREADS, K>;
+    if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
+    const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, total_ranges);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, jobs, count, total_ranges, tables,
+                       ws, reinterpret_cast<uint64_t *>(ws + 4), ws + 4 + 2ull * total_ranges,
+                       out_words, window_per_lane(c), (uint32_t)(HGAP_RANGE_BYTES / tile));
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
+}
+
+static hgap_status launch_enc_range_batch(const EncJobDev *jobs, uint32_t count,
+                                          uint32_t total_ranges, const uint32_t *tables,
+                                          uint32_t *ws, uint32_t max_len,
+                                          const hgap_launch_config &c, hipStream_t s) {
+#define HGAP_ENC_CASE(T, K)                                                                    \
+    if (c.enc_threads == T && c.enc_bytes_per_thread == K)                                   \
+        return launch_enc_range_batch_t<T, K>(jobs, count, total_ranges, tables, ws, max_len,  \
+                                              c, s);
+    HGAP_ENC_CASE(256, 16) HGAP_ENC_CASE(256, 32) HGAP_ENC_CASE(256, 64)
+    HGAP_ENC_CASE(512, 16) HGAP_ENC_CASE(512, 32) HGAP_ENC_CASE(512, 64)
+    HGAP_ENC_CASE(1024, 16) HGAP_ENC_CASE(1024, 32) HGAP_ENC_CASE(1024, 64)
+#undef HGAP_ENC_CASE
+    return HGAP_ERR_UNSUPPORTED;
+}
+
 // Workspace of a batched decode: the offsets kernel's ticket and status words, one count byte
 // per segment, one sum per 64 segments, one 64-bit prefix per tile (of the whole batch).
 struct DecBatchWork {
@@ -2149,11 +2328,34 @@ hgap_status hgap_decode(const uint32_t *d_payload, uint32_t payload_words, const
 
 uint64_t hgap_batch_plan_histogram(hgap_hist_job *jobs, uint32_t count) {
     uint64_t base = 0;
+    uint32_t ranges = 0;
     for (uint32_t j = 0; j < count; j++) {
         jobs[j].base = base;
+        jobs[j].range_base = ranges;
         base += jobs[j].n;
+        ranges += hgap_range_count(jobs[j].n);
     }
     return base;
+}
+
+hgap_status hgap_batch_histogram_ranges(const hgap_hist_job *d_jobs, uint32_t count,
+                                        uint32_t total_ranges, uint32_t *d_range_hist,
+                                        hgap_stream_t stream) {
+    if (count == 0) return HGAP_OK;
+    if (!d_jobs || (total_ranges > 0 && (!d_range_hist || !aligned16(d_range_hist))))
+        return HGAP_ERR_INVALID_ARG;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    (void)hipGetLastError();
+    const HistJobDev *jobs = reinterpret_cast<const HistJobDev *>(d_jobs);
+    hipLaunchKernelGGL(hist_batch_zero_kernel, dim3(count < 4096 ? count : 4096), dim3(256), 0, s,
+                       jobs, count);
+    if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
+    if (total_ranges == 0) return HGAP_OK;
+    const unsigned blocks =
+        grid_for(hist_range_batch_kernel, kHistThreadsSingle, 0, 0, total_ranges);
+    hipLaunchKernelGGL(hist_range_batch_kernel, dim3(blocks), dim3(kHistThreadsSingle), 0, s,
+                       jobs, count, total_ranges, (uint32_t)HGAP_RANGE_BYTES, d_range_hist);
+    return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
 
 hgap_status hgap_batch_histogram(const hgap_hist_job *d_jobs, uint32_t count, uint64_t total,
@@ -2197,6 +2399,19 @@ uint32_t hgap_batch_plan_encode(hgap_enc_job *jobs, uint32_t count,
     return (uint32_t)base;
 }
 
+uint32_t hgap_batch_plan_encode_ranges(hgap_enc_job *jobs, uint32_t count,
+                                       const hgap_launch_config *cfg) {
+    const hgap_launch_config c = resolve(cfg);
+    const uint64_t tile = (uint64_t)c.enc_threads * c.enc_bytes_per_thread;
+    uint64_t base = 0;
+    for (uint32_t j = 0; j < count; j++) {
+        jobs[j].base = (uint32_t)base;
+        jobs[j].num_tiles = (uint32_t)((jobs[j].n + tile - 1) / tile);
+        base += hgap_range_count(jobs[j].n);
+    }
+    return (uint32_t)base;
+}
+
 uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
                                 const hgap_launch_config *cfg) {
     const hgap_launch_config c = resolve(cfg);
@@ -2211,6 +2426,10 @@ uint32_t hgap_batch_plan_decode(hgap_dec_job *jobs, uint32_t count,
 }
 
 size_t hgap_batch_workspace_bytes(uint32_t total_tiles) { return 16 + 8ull * total_tiles; }
+
+size_t hgap_batch_encode_ranges_workspace_bytes(uint32_t total_ranges) {
+    return 16 + 12ull * total_ranges;
+}
 
 size_t hgap_batch_decode_workspace_bytes(uint32_t total_tiles, const hgap_launch_config *cfg) {
     const hgap_launch_config c = resolve(cfg);
@@ -2238,6 +2457,35 @@ hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32
                        dim3(256), 0, s, jobs, count);
     if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
     return launch_enc_batch(jobs, count, total_tiles, d_tables, ws, max_len, c, s);
+}
+
+hgap_status hgap_batch_encode_ranges(const hgap_enc_job *d_jobs, uint32_t count,
+                                     uint32_t total_ranges, uint32_t max_len,
+                                     const uint32_t *d_tables, const uint32_t *d_range_hist,
+                                     void *d_workspace, size_t workspace_bytes,
+                                     const hgap_launch_config *cfg, hgap_stream_t stream) {
+    const hgap_launch_config c = resolve(cfg);
+    if (!config_valid(c)) return HGAP_ERR_INVALID_ARG;
+    if (count == 0 || total_ranges == 0) return HGAP_OK;
+    if (!d_jobs || !d_tables || !d_range_hist || !d_workspace || !aligned16(d_workspace) ||
+        !aligned16(d_range_hist))
+        return HGAP_ERR_INVALID_ARG;
+    if (max_len < 1 || max_len > HGAP_MAX_CODE_LEN) return HGAP_ERR_CODEBOOK;
+    if (workspace_bytes < hgap_batch_encode_ranges_workspace_bytes(total_ranges))
+        return HGAP_ERR_WORKSPACE;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    uint32_t *ws = static_cast<uint32_t *>(d_workspace);
+    const EncJobDev *jobs = reinterpret_cast<const EncJobDev *>(d_jobs);
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(gap_batch_zero_kernel<EncJobDev>, dim3(count < 4096 ? count : 4096),
+                       dim3(256), 0, s, jobs, count);
+    // workspace: counter (16 bytes), a status word per range, the job of each range
+    const uint32_t blocks = std::min<uint32_t>((total_ranges + 3) / 4, 4096);
+    hipLaunchKernelGGL(enc_range_prep_batch_kernel, dim3(blocks), dim3(256), 0, s, jobs, count,
+                       total_ranges, d_tables, d_range_hist, (uint32_t)HGAP_RANGE_BYTES,
+                       reinterpret_cast<uint64_t *>(ws + 4), ws + 4 + 2ull * total_ranges, ws);
+    if (hipGetLastError() != hipSuccess) return HGAP_ERR_HIP;
+    return launch_enc_range_batch(jobs, count, total_ranges, d_tables, ws, max_len, c, s);
 }
 
 hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32_t total_tiles,

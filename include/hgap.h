@@ -197,7 +197,9 @@ typedef struct hgap_hist_job {
     const void *in;  /* device bytes, 16-byte aligned */
     uint64_t n;
     uint32_t *hist;  /* device u32[256], overwritten */
-    uint32_t reserved[2];
+    uint32_t range_base; /* set by hgap_batch_plan_histogram: the job's first row of range
+                          * counts (hgap_batch_histogram_ranges) */
+    uint32_t reserved;
     uint64_t base;   /* set by hgap_batch_plan_histogram: offset in the concatenated batch */
 } hgap_hist_job;
 
@@ -210,8 +212,10 @@ typedef struct hgap_enc_job {
     uint32_t gap_words;
     uint32_t table;         /* index of this job's 256-word code table (hgap_encode_table) */
     uint32_t max_len;       /* longest code length of its codebook */
-    uint32_t base;          /* set by hgap_batch_plan_encode */
-    uint32_t num_tiles;     /* set by hgap_batch_plan_encode */
+    uint32_t base;          /* set by hgap_batch_plan_encode(_ranges) */
+    uint32_t num_tiles;     /* set by hgap_batch_plan_encode(_ranges) */
+    uint32_t range_base;    /* hgap_batch_encode_ranges: the job's first row of range counts */
+    uint32_t reserved;
 } hgap_enc_job;
 
 typedef struct hgap_dec_job {
@@ -232,10 +236,16 @@ typedef struct hgap_dec_job {
     uint32_t reserved;
 } hgap_dec_job;
 
-/* Sets each job's base; returns the total byte count of the batch. */
+/* Sets each job's base and range_base; returns the total byte count of the batch. */
 uint64_t hgap_batch_plan_histogram(hgap_hist_job *jobs, uint32_t count);
 hgap_status hgap_batch_histogram(const hgap_hist_job *d_jobs, uint32_t count, uint64_t total,
                                  hgap_stream_t stream);
+/* hgap_batch_histogram plus per-range counts: row range_base + r of d_range_hist (u32[256]
+ * each) counts range r of the job (HGAP_RANGE_BYTES each, the last one ragged).
+ * total_ranges = the sum of hgap_range_count(n) over the jobs. */
+hgap_status hgap_batch_histogram_ranges(const hgap_hist_job *d_jobs, uint32_t count,
+                                        uint32_t total_ranges, uint32_t *d_range_hist,
+                                        hgap_stream_t stream);
 
 /* Codebooks of `count` histograms (count x 256 u32) on `threads` host threads (0 = all). */
 hgap_status hgap_codebook_build_batch(const uint32_t *hists, uint32_t count, hgap_codebook *cbs,
@@ -272,6 +282,21 @@ hgap_status hgap_batch_encode(const hgap_enc_job *d_jobs, uint32_t count, uint32
                               uint32_t max_len, const uint32_t *d_tables, void *d_workspace,
                               size_t workspace_bytes, const hgap_launch_config *cfg,
                               hgap_stream_t stream);
+/* The chained batched encode (hgap_encode_ranges for every job): hgap_batch_plan_encode_ranges
+ * sets base (the job's first range in this call) and num_tiles and returns the call's total
+ * ranges; each job's range_base points at its rows of d_range_hist, from
+ * hgap_batch_histogram_ranges over the same inputs.  The same streams as hgap_batch_encode;
+ * a range's first tile finds its offset without waiting on another workgroup.  Workspace:
+ * hgap_batch_encode_ranges_workspace_bytes(total_ranges) (a status word and a job index per
+ * range). */
+uint32_t hgap_batch_plan_encode_ranges(hgap_enc_job *jobs, uint32_t count,
+                                       const hgap_launch_config *cfg);
+size_t hgap_batch_encode_ranges_workspace_bytes(uint32_t total_ranges);
+hgap_status hgap_batch_encode_ranges(const hgap_enc_job *d_jobs, uint32_t count,
+                                     uint32_t total_ranges, uint32_t max_len,
+                                     const uint32_t *d_tables, const uint32_t *d_range_hist,
+                                     void *d_workspace, size_t workspace_bytes,
+                                     const hgap_launch_config *cfg, hgap_stream_t stream);
 hgap_status hgap_batch_decode(const hgap_dec_job *d_jobs, uint32_t count, uint32_t total_tiles,
                               uint32_t max_table_words, uint32_t max_symbols, void *d_workspace,
                               size_t workspace_bytes, const hgap_launch_config *cfg,

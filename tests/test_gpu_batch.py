@@ -42,6 +42,21 @@ def test_batched_histograms(batch_data):
         assert (h[i] == np.bincount(b.cpu().numpy(), minlength=256)).all(), i
 
 
+def test_batched_range_histograms(batch_data):
+    """Every buffer's histogram and the counts of each of its 256 KiB ranges (ragged last
+    range, one-range buffers) equal numpy's."""
+    batch, codec, arena, bufs = batch_data
+    h, rh, base = batch.histograms_ranges(bufs)
+    rows = rh.cpu().numpy().view(np.uint32)
+    R = codec.RANGE_BYTES
+    for i, b in enumerate(bufs):
+        d = b.cpu().numpy()
+        assert (h[i] == np.bincount(d, minlength=256)).all(), i
+        for r in range(-(-d.size // R)):
+            want = np.bincount(d[r * R:(r + 1) * R], minlength=256)
+            assert (rows[base[i] + r] == want).all(), (i, r)
+
+
 def test_batched_codebooks_match_single(batch_data, orc):
     batch, codec, arena, bufs = batch_data
     h = batch.histograms(bufs)
@@ -51,8 +66,11 @@ def test_batched_codebooks_match_single(batch_data, orc):
         assert list(cbs[i].code) == list(o["code"]) and list(cbs[i].length) == list(o["length"])
 
 
+@pytest.mark.parametrize("chained", [False, True])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_batched_encode_decode_bit_exact(batch_data, orc, mixed):
+def test_batched_encode_decode_bit_exact(batch_data, orc, mixed, chained):
+    """Per-tile look-back (chained=False) and chained encode launches (chained=True: the range
+    counts of the batched histogram give every range's offset) write the oracle's streams."""
     import torch
 
     from huffman_coding_gap_arrays_amd import tuner
@@ -69,8 +87,9 @@ def test_batched_encode_decode_bit_exact(batch_data, orc, mixed):
         for c in configs:
             if c["enc_threads"] * c["enc_bytes_per_thread"] > 32768:
                 c["enc_bytes_per_thread"] = 32
-    say("batch codec", "mixed" if mixed else "default")
-    bc = batch.BatchCodec(bufs, configs=configs)
+    say("batch codec", "mixed" if mixed else "default", "chained" if chained else "look-back")
+    bc = batch.BatchCodec(bufs, configs=configs, chained=chained)
+    assert all(L["chained"] == chained for L in bc.enc_launches)
     bc.run()
     torch.cuda.synchronize()
     for i, b in enumerate(bufs):
@@ -119,13 +138,14 @@ def test_batched_decode_through_small_image(batch_data, monkeypatch):
     assert bc.mismatches() == 0
 
 
-def test_batched_graph_replays(batch_data):
+@pytest.mark.parametrize("chained", [False, True])
+def test_batched_graph_replays(batch_data, chained):
     """The captured graph re-runs encode + decode exactly, replay after replay, with eager work
     and synchronizations in between."""
     import torch
 
     batch, codec, arena, bufs = batch_data
-    bc = batch.BatchCodec(bufs)
+    bc = batch.BatchCodec(bufs, chained=chained)
     bc.run()
     torch.cuda.synchronize()
     ref_pay = bc.payload.mem.clone()

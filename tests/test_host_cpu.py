@@ -255,6 +255,13 @@ def test_device_entry_points_validate_before_launch(capi):
                                    ctypes.c_void_p(24), q, q, q, 4096, None, None) == 1
     assert capi.hgap_encode_ranges(q, 100, ctypes.byref(cb), ctypes.byref(sz), q, q, q, q, 8,
                                    None, None) == 4  # workspace too small
+    # batched per-range counts and the chained batched encoder
+    assert capi.hgap_batch_histogram_ranges(q, 1, 5, None, None) == 1
+    assert capi.hgap_batch_histogram_ranges(q, 1, 5, ctypes.c_void_p(20), None) == 1
+    assert capi.hgap_batch_encode_ranges_workspace_bytes(10) == 16 + 10 * 12
+    assert capi.hgap_batch_encode_ranges(q, 1, 10, 8, q, None, q, 4096, None, None) == 1
+    assert capi.hgap_batch_encode_ranges(q, 1, 10, 0, q, q, q, 4096, None, None) == 3
+    assert capi.hgap_batch_encode_ranges(q, 1, 10, 8, q, q, q, 135, None, None) == 4
     assert capi.hgap_encode_workspace_bytes(0) == 16
     assert capi.hgap_encode_workspace_bytes(8193) == 16 + 3 * 8  # smallest tile: 4 KiB
     # ticket + status word of one offsets block, one count byte per segment (whole 1024-segment
@@ -408,13 +415,21 @@ def test_batch_job_layout_and_plans(capi, golden_codebooks):
     from huffman_coding_gap_arrays_amd import batch
     from huffman_coding_gap_arrays_amd._capi import Codebook, LaunchConfig, default_config
 
-    assert (batch.HIST_JOB.itemsize, batch.ENC_JOB.itemsize, batch.DEC_JOB.itemsize) == (40, 56, 80)
+    assert (batch.HIST_JOB.itemsize, batch.ENC_JOB.itemsize, batch.DEC_JOB.itemsize) == (40, 64, 80)
     rng = np.random.default_rng(2)
     ns = rng.integers(0, 3 << 20, 50)
+    ns[[3, 17]] = 0  # empty buffers own no ranges
     jobs = np.zeros(50, batch.HIST_JOB)
     jobs["n"] = ns
     total = capi.hgap_batch_plan_histogram(vp(jobs), 50)
     assert total == ns.sum() and (jobs["base"] == np.concatenate([[0], np.cumsum(ns)[:-1]])).all()
+    rows = -(-ns // (256 * 1024))
+    assert (jobs["range_base"] == np.concatenate([[0], np.cumsum(rows)[:-1]])).all()
+    ej = np.zeros(50, batch.ENC_JOB)
+    ej["n"] = ns
+    ranges = capi.hgap_batch_plan_encode_ranges(vp(ej), 50, None)  # default 32 KiB tiles
+    assert ranges == rows.sum() and (ej["base"] == jobs["range_base"]).all()
+    assert (ej["num_tiles"] == -(-ns // 32768)).all()
     cfg = LaunchConfig.from_dict({**default_config().as_dict(), "enc_threads": 256,
                                   "enc_bytes_per_thread": 32, "dec_threads": 512})
     ej = np.zeros(50, batch.ENC_JOB)
