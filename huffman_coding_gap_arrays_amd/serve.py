@@ -72,18 +72,25 @@ class Request:
         return None if self.done is None else self.done - self.arrival
 
 
+def _range_bytes(n):
+    """Per-range counts (1 KiB per 256 KiB range) and the chained encoder's status word and job
+    index per range."""
+    return 1036 * (-(-n // codec.RANGE_BYTES))
+
+
 def encode_reserve_bound(n):
     """Device bytes an encode request may need before its histogram is known: payload for codes
     of at most 16 bits, the gap array, look-back status words for the smallest tile, its
-    histogram, and allocation rounding."""
+    histogram and range counts, and allocation rounding."""
     words = (16 * n + 31) // 32
     gap_words = ((16 * n + 127) // 128 + 7) // 8
-    return 4 * words + 4 * gap_words + 8 * (n // 4096 + 2) + 1024 + 4 * ALIGN
+    return (4 * words + 4 * gap_words + 8 * (n // 4096 + 2) + _range_bytes(n) + 1024
+            + 4 * ALIGN)
 
 
 def encode_reserve_exact(n, sizes):
     return (4 * int(sizes.payload_words) + 4 * int(sizes.gap_words) + 8 * (n // 4096 + 2)
-            + 1024 + 4 * ALIGN)
+            + _range_bytes(n) + 1024 + 4 * ALIGN)
 
 
 def decode_reserve(n, gap_elements):
@@ -214,11 +221,16 @@ class _Batch:
             jobs[j]["in_"] = r.data.data_ptr()
             jobs[j]["n"] = r.n
             jobs[j]["hist"] = self.hist.data_ptr() + 1024 * j
-        total = lib.hgap_batch_plan_histogram(_vp(jobs), len(self.enc))
+        lib.hgap_batch_plan_histogram(_vp(jobs), len(self.enc))
+        # per-range counts too, for encode launches that take the chained encoder
+        self.range_base = jobs["range_base"].astype(np.uint32).copy()
+        ranges = int((-(-jobs["n"].astype(np.int64) // codec.RANGE_BYTES)).sum())
+        self.range_hist = torch.empty((max(ranges, 1), 256), dtype=torch.int32, device=s.device)
         host, d_jobs = _pinned_to_device(jobs, s.device)
-        self.keep += [host, d_jobs]
-        check(lib.hgap_batch_histogram(ctypes.c_void_p(d_jobs.data_ptr()), len(self.enc), total,
-                                       _sp(self.stream)), "hgap_batch_histogram")
+        self.keep += [host, d_jobs, self.range_hist]
+        check(lib.hgap_batch_histogram_ranges(ctypes.c_void_p(d_jobs.data_ptr()), len(self.enc),
+                                              ranges, ctypes.c_void_p(self.range_hist.data_ptr()),
+                                              _sp(self.stream)), "hgap_batch_histogram_ranges")
         self.hist_host = torch.empty((len(self.enc), 256), dtype=torch.int32, pin_memory=True)
         self.hist_host.copy_(self.hist, non_blocking=True)
         self.hist_event = torch.cuda.Event()
@@ -266,17 +278,33 @@ class _Batch:
                     jobs[q]["table"] = j
                     ml = max(e.codebook.length[:]) or 1
                     jobs[q]["max_len"] = ml
+                    jobs[q]["range_base"] = self.range_base[j]
                     max_len = max(max_len, ml)
-                total = lib.hgap_batch_plan_encode(_vp(jobs), len(members), ctypes.byref(cfg))
+                ranges = int((-(-jobs["n"].astype(np.int64) // codec.RANGE_BYTES)).sum())
+                if ranges >= s.chained_min_ranges:
+                    total = lib.hgap_batch_plan_encode_ranges(_vp(jobs), len(members),
+                                                              ctypes.byref(cfg))
+                    wsb = lib.hgap_batch_encode_ranges_workspace_bytes(total)
+                else:
+                    total = lib.hgap_batch_plan_encode(_vp(jobs), len(members), ctypes.byref(cfg))
+                    wsb = lib.hgap_batch_workspace_bytes(total)
                 host, d_jobs = _pinned_to_device(jobs, s.device)
-                ws = torch.empty(_up(16, lib.hgap_batch_workspace_bytes(total)),
-                                 dtype=torch.uint8, device=s.device)
+                ws = torch.empty(_up(16, wsb), dtype=torch.uint8, device=s.device)
                 self.keep += [host, d_jobs, ws, cfg]
-                check(lib.hgap_batch_encode(ctypes.c_void_p(d_jobs.data_ptr()), len(members),
-                                            total, max_len, ctypes.c_void_p(d_tables.data_ptr()),
-                                            ctypes.c_void_p(ws.data_ptr()), ws.numel(),
-                                            ctypes.byref(cfg), _sp(self.stream)),
-                      "hgap_batch_encode")
+                if ranges >= s.chained_min_ranges:
+                    check(lib.hgap_batch_encode_ranges(
+                        ctypes.c_void_p(d_jobs.data_ptr()), len(members), total, max_len,
+                        ctypes.c_void_p(d_tables.data_ptr()),
+                        ctypes.c_void_p(self.range_hist.data_ptr()),
+                        ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.byref(cfg),
+                        _sp(self.stream)), "hgap_batch_encode_ranges")
+                else:
+                    check(lib.hgap_batch_encode(ctypes.c_void_p(d_jobs.data_ptr()), len(members),
+                                                total, max_len,
+                                                ctypes.c_void_p(d_tables.data_ptr()),
+                                                ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                                ctypes.byref(cfg), _sp(self.stream)),
+                          "hgap_batch_encode")
             self._finish_launches()
 
     def _finish_launches(self):
@@ -296,8 +324,12 @@ class Server:
     """Continuous-batching request loop on one device."""
 
     def __init__(self, cap_bytes=200 * 10**9, max_batch=64, max_inflight=4, streams=2,
-                 device="cuda", predictor=None, codebook_threads=0, config=None):
+                 device="cuda", predictor=None, codebook_threads=0, config=None,
+                 chained_min_ranges=batch.BatchCodec.CHAINED_MIN_RANGES):
+        """An encode launch whose requests hold at least `chained_min_ranges` 256 KiB ranges
+        takes the chained encoder (offsets from the range counts of the batched histogram)."""
         self.lib = _capi.load()
+        self.chained_min_ranges = int(chained_min_ranges)
         self.device = device
         self.cap = int(cap_bytes)
         self.max_batch = int(max_batch)

@@ -63,12 +63,16 @@ def test_encode_responses_match_oracle(reqs, alone, orc):
         assert np.array_equal(got_gap, gap), r.rid
 
 
-@pytest.mark.parametrize("streams,inflight,batch", [(2, 3, 8), (3, 4, 64), (1, 2, 4)])
-def test_same_alone_and_in_the_mix(reqs, alone, streams, inflight, batch):
+@pytest.mark.parametrize("streams,inflight,batch,chained_min",
+                         [(2, 3, 8, 512), (3, 4, 64, 0), (1, 2, 4, 512)])
+def test_same_alone_and_in_the_mix(reqs, alone, streams, inflight, batch, chained_min):
+    """chained_min = 0: every encode launch takes the chained encoder (the requests alone take
+    the per-tile look-back): the same bits."""
     needs = [serve.request_bytes(r) for r in reqs]
     cap = int(2.5 * max(needs))  # tight: admission has to wait for memory
-    say("mix", streams, inflight, batch, "cap", cap)
-    srv = serve.Server(cap_bytes=cap, max_batch=batch, max_inflight=inflight, streams=streams)
+    say("mix", streams, inflight, batch, chained_min, "cap", cap)
+    srv = serve.Server(cap_bytes=cap, max_batch=batch, max_inflight=inflight, streams=streams,
+                       chained_min_ranges=chained_min)
     serve.reset(reqs, serve.poisson_arrivals(len(reqs), 200000.0, seed=streams))
     st = srv.run(reqs)
     assert st["served"] == len(reqs) and st["rejected"] == 0
