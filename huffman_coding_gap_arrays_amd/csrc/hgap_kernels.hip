@@ -2,17 +2,23 @@
 //
 //   hist_kernel     byte histogram; one private 256-bin array per lane index in LDS, merged once
 //                   per workgroup into the global histogram
+//   hist_range_kernel
+//                   the same, plus the counts of every 256 KiB range of the input
 //   enc_kernel      persistent encoder: per-thread code lengths, workgroup scan, single-pass
 //                   decoupled look-back across tiles (64-bit status words in HBM), bit-packing
-//                   into an LDS tile with gap nibbles emitted on the fly, 128-bit stores
+//                   into an LDS tile with gap nibbles emitted on the fly, 128-bit stores.
+//                   Chained form (after enc_range_prep_kernel): a workgroup encodes a range's
+//                   tiles in order from the range's offset, which the range counts give in
+//                   advance, so no tile waits on another workgroup
 //   dec_count_kernel, dec_offsets_kernel, dec_write_kernel
 //                   decoder in three launches: per-segment codeword counts (one lane per
 //                   128-bit segment from its gap offset, two-level table in LDS), per-tile output
 //                   offsets, then a second decode of each segment into an LDS image of the
 //                   tile's output and 128-bit stores
+//   hist_batch_kernel, hist_range_batch_kernel, enc_batch_kernel, enc_range_batch_kernel,
 //   dec_count_batch_kernel, dec_write_batch_kernel
-//                   the same decode over many buffers (each with its own table) in one launch
-//                   per step
+//                   the same steps over many buffers (each with its own codebook or table) in
+//                   one launch per step
 //
 // Reference citations are file:line relative to the reference repository root.
 #include <hip/hip_runtime.h>
@@ -706,19 +712,18 @@ __global__ __launch_bounds__(256) void enc_range_prep_kernel(
 // Tile = THREADS consecutive 128-bit segments, one per thread.  Segment s starts at bit
 // 128*s + (gap element s-1) and decodes every codeword that starts inside it (and inside the
 // payload), as in decoder.cu:505-569.  A lane holds its segment's four payload words and the
-// word after them in registers (loaded one tile ahead).  Per tile:
-//   1. walk 1: each lane counts its segment's codewords (64-bit bit buffer, two-level table in
-//      LDS, up to two codewords per lookup);
-//   2. workgroup scan of the counts; one wave publishes the tile's symbol count and issues the
-//      first look-back loads;
-//   3. walk 2: each lane decodes its segment again and writes its symbols straight into the
-//      tile's output image in LDS (byte 16 + i of the image = output byte i of the tile), while
-//      the look-back loads are in flight.  A step stores two bytes whatever the entry holds, so
-//      a lane leaves one stray byte just past its range, on its successor's first byte: after
-//      a barrier every lane writes its first symbol again;
-//   4. the look-back wave resolves the tile's output offset O and claims the next tile;
-//   5. owned output bytes [O, O + count) are written with 128-bit stores (byte stores at the
-//      two shared 16-byte edges); bytes at or past n (decoded zero padding) are dropped.
+// word after them in registers (loaded one tile ahead).  Three launches:
+//   1. dec_count_kernel: walk 1, each lane counts its segment's codewords (64-bit bit buffer,
+//      one- or two-level table in LDS, up to two codewords per lookup) and stores the count
+//      (one byte per segment) and each wave's sum;
+//   2. dec_offsets_kernel: each tile's output offset O, a scan of the wave sums;
+//   3. dec_write_kernel: a workgroup scan of the tile's counts, then walk 2, each lane
+//      decoding its segment again straight into an LDS image of the tile's output (image byte
+//      j = output byte (O & ~15) + j).  A step stores two bytes whatever the entry holds, so a
+//      lane leaves one stray byte on its successor's first byte, which the successor writes
+//      again after its own walk (see dec_write_tile).  Owned output bytes [O, O + count) leave
+//      with 128-bit stores (byte stores at the two shared 16-byte edges); bytes at or past n
+//      (decoded zero padding) are dropped.
 // The image region holds `image_bytes` bytes (chosen by the launcher for occupancy); a tile
 // whose symbols do not fit goes through it in windows.
 struct DecBuffer {
