@@ -136,7 +136,10 @@ hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
 size_t hgap_encode_workspace_bytes(uint64_t n);
 /* Writes payload[W] and gap[ceil(G/8)] for d_in[0..n) with codebook `cb`; sizes must come from
  * hgap_encode_sizes for the same input.  Replaces encode() (encoder/src/encoder.cu:382-457):
- * no host copies, no memset of the payload, the caller keeps ownership of d_in. */
+ * no host copies, no memset of the payload, the caller keeps ownership of d_in.  Launch shape:
+ * hgap_default_config's; from HGAP_CHAINED_MIN_BYTES of input, the workgroup is halved while two
+ * workgroups would not fit a CU's LDS for this codebook's longest code (and at any size, smaller
+ * again when one does not fit). */
 hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
                         const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
