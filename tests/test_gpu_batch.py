@@ -24,6 +24,9 @@ def batch_data(cuda):
     spec = synth.batch_spec(count=60, seed=9, min_bytes=1 << 10, max_bytes=3 << 20)
     spec += [dict(n=n, s=1.2, seed=1000 + n, perm_seed=n) for n in (1, 15, 16, 17, 4095)]
     spec.append(dict(n=20000, s=40.0, seed=77, perm_seed=1))  # one distinct byte
+    # empty buffers (no bytes, no ranges, no tiles) between and after the others
+    spec.insert(30, dict(n=0, s=1.2, seed=3, perm_seed=3))
+    spec.append(dict(n=0, s=1.2, seed=4, perm_seed=4))
     arena = batch.Arena([d["n"] for d in spec])
     bufs = []
     for i, d in enumerate(spec):
