@@ -733,10 +733,20 @@ struct DecBuffer {
     const uint32_t *gap;
     uint8_t *out;
     uint64_t n;
-    uint64_t *status;  // one status word per tile of this buffer
+    uint64_t *status;  // unused by the three-launch decoder
     uint32_t num_tiles;
-    uint32_t reserved;
+    uint32_t last_limit;  // payload bits in the last segment (dec_last_limit)
 };
+
+// Bits of the stream's last segment that lie inside the payload (1..128); every other segment
+// has all 128.
+__host__ __device__ inline uint32_t dec_last_limit(uint32_t payload_words, uint32_t gap_elements) {
+    if (gap_elements == 0) return 0u;
+    const uint64_t bits = 32ull * payload_words;
+    const uint64_t before = 128ull * (gap_elements - 1);
+    if (bits <= before) return 0u;
+    return bits - before < 128 ? (uint32_t)(bits - before) : 128u;
+}
 
 struct DecTable {
     uint32_t prefix_bits;
@@ -810,9 +820,8 @@ __device__ __forceinline__ DecSeg dec_segment(const DecBuffer &buf, const DecSta
     g.w4 = __builtin_amdgcn_ds_bpermute(4 * ((lane_id() + 1) & 63u), (int)st.q[0]);
     if (lane_id() == 63) g.w4 = st.extra;
     g.p0 = seg > 0 ? (st.gapw >> (((seg - 1) & 7u) * 4u)) & 15u : 0u;
-    const uint64_t avail =
-        seg < buf.gap_elements ? 32ull * buf.payload_words - 128ull * seg : 0ull;
-    g.limit = avail < 128 ? (uint32_t)avail : 128u;
+    g.limit = seg + 1 < buf.gap_elements ? 128u
+              : (seg + 1 == buf.gap_elements ? buf.last_limit : 0u);
     return g;
 }
 
@@ -1476,7 +1485,7 @@ __device__ __forceinline__ DecBuffer dec_job_buffer(const DecJobDev &job) {
     buf.n = job.n;
     buf.status = nullptr;
     buf.num_tiles = job.num_tiles;
-    buf.reserved = 0;
+    buf.last_limit = dec_last_limit(job.payload_words, job.gap_elements);
     return buf;
 }
 
@@ -2331,7 +2340,7 @@ hgap_status hgap_decode_cfg(const uint32_t *d_payload, uint32_t payload_words,
     b.n = n;
     b.status = nullptr;
     b.num_tiles = (uint32_t)tiles;
-    b.reserved = 0;
+    b.last_limit = dec_last_limit(payload_words, gap_elements);
     // size the LDS image for this stream's density (1.25 x its mean codewords per segment, + 2)
     // rather than the table's worst case: denser tiles go through the region in windows
     hgap_dectable_info sized = *info;
