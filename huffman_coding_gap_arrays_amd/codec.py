@@ -254,6 +254,15 @@ class DeviceTable:
         self.image = image
         self.table = torch.from_numpy(image.view(np.int32)).to(device)
 
+    @classmethod
+    def resident(cls, info, table):
+        """A table image already in device memory (`table`: int32 tensor, `info` its sizes)."""
+        t = cls.__new__(cls)
+        t.info = info
+        t.image = None
+        t.table = table
+        return t
+
 
 def decode(payload, gap, payload_words, gap_elements, table, n, out=None, stream=None,
            workspace=None, config=None):

@@ -216,14 +216,19 @@ hgap_status hgap_codebook_build(const uint32_t hist[256], hgap_codebook *cb) {
 
 namespace {
 
-// Runs fn(i) for i in [0, count) on `threads` host threads (0 = all cores); returns the first
-// non-OK status any call returned.
+// Runs fn(i) for i in [0, count) on up to `threads` host threads (0 = all cores), at least
+// kItemsPerThread items per thread: starting a thread costs more than several codebooks (a
+// codebook takes about 33 us; 4 of them on 4 new threads took 540 us, on the caller's 205 us).
+// Returns the first non-OK status any call returned.
+constexpr uint32_t kItemsPerThread = 8;
+
 template <typename F>
 hgap_status parallel_for(uint32_t count, uint32_t threads, F fn) {
     if (count == 0) return HGAP_OK;
     const uint32_t hw = std::thread::hardware_concurrency();
     if (threads == 0) threads = hw ? hw : 1;
-    if (threads > count) threads = count;
+    const uint32_t most = (count + kItemsPerThread - 1) / kItemsPerThread;
+    if (threads > most) threads = most;
     std::atomic<uint32_t> next{0};
     std::atomic<int> err{HGAP_OK};
     auto work = [&]() {
