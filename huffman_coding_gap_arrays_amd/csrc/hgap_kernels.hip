@@ -170,12 +170,11 @@ __device__ uint64_t look_back(uint64_t *status, uint32_t tile, uint64_t aggregat
 // ---------------------------------------------------------------------------------------------
 // Where an encoder tile claims the workgroup's next tile: after its own look-back, so tiles are
 // claimed in the order they finish looking back (a tile never waits on a predecessor that has
-// not started), and the next tile's input copy overlaps this tile's packing and stores.
+// not started), and the next tile's input load overlaps this tile's packing and stores.
 struct NextTile {
     uint32_t *counter;  // ticket counter; nullptr = the caller claims tiles itself
     uint32_t *s_ticket; // LDS word broadcasting the ticket
     uint32_t tile;      // out: the claimed ticket
-    uint8_t *s_stage;   // encoder: LDS staging area the next tile's input is copied into
 };
 
 // zero fill (the histogram, the gap array and the look-back workspaces)
@@ -401,38 +400,6 @@ __device__ __forceinline__ uint32_t enc_byte(const EncStage<K> &st, int j) {
     return (st.q[j >> 4][(j >> 2) & 3] >> (8 * (j & 3))) & 0xFFu;
 }
 
-// Whether every input byte of `tile` lies inside the buffer (only such tiles are staged
-// through LDS: the copies move whole 16-byte pieces).
-template <int THREADS, int K>
-__device__ __forceinline__ bool enc_tile_full(const EncBuffer &buf, uint32_t tile) {
-    return tile < buf.num_tiles && (uint64_t)(tile + 1) * (THREADS * K) <= buf.n;
-}
-
-// Starts copying a full tile's input from HBM into LDS without passing through registers
-// (global_load_lds_dwordx4): piece v of thread t lands at s_stage + 16 * (v * THREADS + t), so
-// every lane later reads back exactly the pieces its own load wrote.
-template <int THREADS, int K>
-__device__ __forceinline__ void enc_stage_async(const EncBuffer &buf, uint32_t tile,
-                                                uint8_t *s_stage) {
-    const uint8_t *src = buf.in + (uint64_t)tile * (THREADS * K) + (uint64_t)threadIdx.x * K;
-    const uint32_t wave = threadIdx.x / 64u;
-#pragma unroll
-    for (int v = 0; v < K / 16; v++)
-        __builtin_amdgcn_global_load_lds(src + 16 * v, s_stage + 16u * (v * THREADS + wave * 64u),
-                                         16, 0, 0);
-}
-
-template <int THREADS, int K>
-__device__ __forceinline__ EncStage<K> enc_read_stage(const uint8_t *s_stage) {
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
-    EncStage<K> st;
-#pragma unroll
-    for (int v = 0; v < K / 16; v++)
-        st.q[v] = *reinterpret_cast<const u32x4 *>(s_stage + 16u * (v * THREADS + threadIdx.x));
-    st.nb = K;
-    return st;
-}
-
 // How a chained encoder finds E.  The input is cut into ranges of whole tiles; a workgroup
 // encodes a range's tiles in order.  The first tile of a range looks back over per-range status
 // words whose aggregates (the ranges' bit counts, from per-range histograms) were all stored
@@ -451,12 +418,13 @@ struct EncChain {
     uint32_t carry;            // in: bits of E's word before E (when !first); out: the next's
 };
 
-template <int THREADS, int K, bool CHAINED>
+// PREFETCH: load the next tile's input bytes into `ahead` once the next tile is known.
+template <int THREADS, int K, bool CHAINED, bool PREFETCH>
 __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
                                          const EncStage<K> &st, const uint32_t *s_tab,
                                          uint32_t *s_out, uint32_t *s_gap, uint32_t *s_wave,
                                          uint64_t *s_excl, uint32_t wpl, NextTile &next,
-                                         EncChain &ch) {
+                                         EncChain &ch, EncStage<K> &ahead) {
     constexpr uint32_t TILE = THREADS * K;
     const uint32_t lane = lane_id();
     const uint8_t *in = buf.in;
@@ -511,13 +479,13 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
         next.tile = tile + 1 < ch.end ? tile + 1
                     : (ch.next_range < ch.ranges ? ch.next_range * ch.tiles_per_range
                                                  : buf.num_tiles);
-        if (next.s_stage && enc_tile_full<THREADS, K>(buf, next.tile))
-            enc_stage_async<THREADS, K>(buf, next.tile, next.s_stage);
     } else if (next.counter) {
         next.tile = *next.s_ticket;
-        if (next.s_stage && enc_tile_full<THREADS, K>(buf, next.tile))
-            enc_stage_async<THREADS, K>(buf, next.tile, next.s_stage);
     }
+    // the next tile's input bytes into registers now, so that their HBM latency overlaps this
+    // tile's packing and stores.  (Copying them HBM->LDS with global_load_lds instead made the
+    // compiler wait for the whole copy before this tile's first LDS access.)
+    if (PREFETCH) ahead = enc_fetch<THREADS, K>(buf, next.tile);
     const uint64_t base_w = (E >> 5) & ~3ull;  // global word of s_out[0], 4 x global segment E/128
     const uint64_t gap_base = E >> 10;         // global gap word of bytes 0..7 of s_gap
     // one spill byte per segment: byte (E/128 mod 8) + i of s_gap is segment E/128 + i
@@ -620,20 +588,15 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     __syncthreads();
 }
 
-// LDS words before the input staging area of the plain encoder (16-byte aligned)
-__host__ __device__ constexpr uint32_t enc_stage_word(uint32_t out_words, uint32_t gap_words) {
-    return (out_words + gap_words + 3u) & ~3u;
-}
-
-// STAGE: the next tile's input is copied into LDS (a tile's worth of bytes after the images)
-// while the current tile packs and stores, claimed right after the current tile's look-back.
-// CHAINED: the counter hands out ranges of `tiles_per_range` tiles (buf.status has one word per
-// range, prepared by enc_range_prep_kernel); otherwise single tiles.
-template <int THREADS, int K, bool STAGE, bool CHAINED>
+// The next tile's input is loaded into registers one tile ahead (claimed right after the
+// current tile's look-back).  CHAINED: the counter hands out ranges of `tiles_per_range` tiles
+// (buf.status has one word per range, prepared by enc_range_prep_kernel); otherwise single
+// tiles.
+template <int THREADS, int K, bool CHAINED>
 __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable table,
                                                       uint32_t *__restrict__ counter,
-                                                      uint32_t out_words, uint32_t stage_word,
-                                                      uint32_t wpl, uint32_t tiles_per_range) {
+                                                      uint32_t out_words, uint32_t wpl,
+                                                      uint32_t tiles_per_range) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_tab[256];
     __shared__ uint32_t s_wave[THREADS / 64];
@@ -641,7 +604,6 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
     __shared__ uint32_t s_tile;
     uint32_t *s_out = s_dyn;
     uint32_t *s_gap = s_dyn + out_words;
-    uint8_t *s_stage = reinterpret_cast<uint8_t *>(s_dyn + stage_word);
     for (int i = threadIdx.x; i < 256; i += THREADS) s_tab[i] = table.e[i];
     if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
     __syncthreads();
@@ -653,9 +615,8 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
         ch.next_range = s_tile;
         first_tile = s_tile < ch.ranges ? s_tile * tiles_per_range : buf.num_tiles;
     }
-    NextTile next{counter, &s_tile, first_tile, STAGE ? s_stage : nullptr};
-    if (STAGE && enc_tile_full<THREADS, K>(buf, next.tile))
-        enc_stage_async<THREADS, K>(buf, next.tile, s_stage);
+    NextTile next{counter, &s_tile, first_tile};
+    EncStage<K> ahead = enc_fetch<THREADS, K>(buf, next.tile);
     if (CHAINED) {
         while (ch.next_range < ch.ranges) {
             ch.range = ch.next_range;
@@ -663,22 +624,18 @@ __global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable ta
             ch.end = lo + tiles_per_range < buf.num_tiles ? lo + tiles_per_range : buf.num_tiles;
             for (uint32_t cur = lo; cur < ch.end; cur++) {
                 ch.first = cur == lo;
-                const EncStage<K> st = (STAGE && enc_tile_full<THREADS, K>(buf, cur))
-                                           ? enc_read_stage<THREADS, K>(s_stage)
-                                           : enc_fetch<THREADS, K>(buf, cur);
-                enc_tile<THREADS, K, true>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl,
-                                           wpl, next, ch);
+                const EncStage<K> st = ahead;
+                enc_tile<THREADS, K, true, true>(buf, cur, st, s_tab, s_out, s_gap, s_wave,
+                                                 &s_excl, wpl, next, ch, ahead);
             }
         }
         return;
     }
     while (next.tile < buf.num_tiles) {
         const uint32_t cur = next.tile;
-        const EncStage<K> st = (STAGE && enc_tile_full<THREADS, K>(buf, cur))
-                                   ? enc_read_stage<THREADS, K>(s_stage)
-                                   : enc_fetch<THREADS, K>(buf, cur);
-        enc_tile<THREADS, K, false>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
-                                    next, ch);
+        const EncStage<K> st = ahead;
+        enc_tile<THREADS, K, false, true>(buf, cur, st, s_tab, s_out, s_gap, s_wave, &s_excl,
+                                          wpl, next, ch, ahead);
     }
 }
 
@@ -1330,10 +1287,11 @@ __global__ __launch_bounds__(THREADS) void enc_batch_kernel(
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
         const EncStage<K> st = enc_fetch<THREADS, K>(buf, g - job.base);
-        NextTile none{nullptr, nullptr, 0u, nullptr};
+        NextTile none{nullptr, nullptr, 0u};
         EncChain unused{};
-        enc_tile<THREADS, K, false>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave, &s_excl,
-                                    wpl, none, unused);
+        EncStage<K> no_prefetch;
+        enc_tile<THREADS, K, false, false>(buf, g - job.base, st, s_tab, s_out, s_gap, s_wave,
+                                           &s_excl, wpl, none, unused, no_prefetch);
     }
 }
 
@@ -1391,7 +1349,7 @@ __global__ __launch_bounds__(THREADS) void enc_range_batch_kernel(
     EncChain ch{};
     ch.next_range = s_ticket;
     ch.tiles_per_range = tiles_per_range;
-    NextTile next{counter, &s_ticket, 0u, nullptr};
+    NextTile next{counter, &s_ticket, 0u};
     uint32_t loaded = 0xFFFFFFFFu;
     while (ch.next_range < total_ranges) {  // uniform
         const uint32_t j = range_job[ch.next_range];
@@ -1412,7 +1370,7 @@ __global__ __launch_bounds__(THREADS) void enc_range_batch_kernel(
         buf.num_tiles = job.num_tiles;
         buf.reserved = 0;
         ch.range = ch.next_range - job.base;
-        ch.ranges = (job.num_tiles + tiles_per_range - 1) / tiles_per_range;
+        ch.ranges = 0;  // the range's last tile prefetches nothing (the next range is another's)
         const uint32_t lo = ch.range * tiles_per_range;
         if (lo >= job.num_tiles) {  // a range without tiles (only when the plan does not match)
             __syncthreads();
@@ -1422,11 +1380,12 @@ __global__ __launch_bounds__(THREADS) void enc_range_batch_kernel(
             continue;
         }
         ch.end = lo + tiles_per_range < job.num_tiles ? lo + tiles_per_range : job.num_tiles;
+        EncStage<K> ahead = enc_fetch<THREADS, K>(buf, lo);
         for (uint32_t t = lo; t < ch.end; t++) {
             ch.first = t == lo;
-            const EncStage<K> st = enc_fetch<THREADS, K>(buf, t);
-            enc_tile<THREADS, K, true>(buf, t, st, s_tab, s_out, s_gap, s_wave, &s_excl, wpl,
-                                       next, ch);
+            const EncStage<K> st = ahead;
+            enc_tile<THREADS, K, true, true>(buf, t, st, s_tab, s_out, s_gap, s_wave, &s_excl,
+                                             wpl, next, ch, ahead);
         }
     }
 }
@@ -1751,20 +1710,10 @@ static hgap_status launch_enc_kernel(const EncBuffer &b, const EncTable &t, uint
     const size_t lds = 4ull * (out_words + enc_gap_words(tile, max_len));
     const uint64_t units =
         CHAINED ? (b.num_tiles + tiles_per_range - 1) / tiles_per_range : b.num_tiles;
-    // with the input staging area when it fits the CU's LDS, else without
-    const uint32_t stage_word = enc_stage_word(out_words, enc_gap_words(tile, max_len));
-    const size_t lds_staged = 4ull * stage_word + tile;
-    auto staged = enc_kernel<THREADS, K, true, CHAINED>;
-    if (allow_dynamic_lds(staged, lds_staged)) {
-        const unsigned grid = grid_for(staged, THREADS, lds_staged, c.blocks_per_cu, units);
-        hipLaunchKernelGGL(staged, dim3(grid), dim3(THREADS), lds_staged, s, b, t, counter,
-                           out_words, stage_word, window_per_lane(c), tiles_per_range);
-        return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
-    }
-    auto kern = enc_kernel<THREADS, K, false, CHAINED>;
+    auto kern = enc_kernel<THREADS, K, CHAINED>;
     if (!allow_dynamic_lds(kern, lds)) return HGAP_ERR_UNSUPPORTED;
     const unsigned grid = grid_for(kern, THREADS, lds, c.blocks_per_cu, units);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words, 0u,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(THREADS), lds, s, b, t, counter, out_words,
                        window_per_lane(c), tiles_per_range);
     return hipGetLastError() == hipSuccess ? HGAP_OK : HGAP_ERR_HIP;
 }
@@ -2067,7 +2016,7 @@ template <int THREADS, int K>
 static bool enc_shape_fits(uint32_t max_len) {
     constexpr uint32_t tile = THREADS * K;
     const size_t lds = 4ull * (enc_out_words(tile, max_len) + enc_gap_words(tile, max_len));
-    return lds_fits(enc_kernel<THREADS, K, false, false>, lds) &&
+    return lds_fits(enc_kernel<THREADS, K, false>, lds) &&
            lds_fits(enc_batch_kernel<THREADS, K>, lds);
 }
 
@@ -2161,32 +2110,6 @@ hgap_status hgap_histogram_ranges(const void *d_in, uint64_t n, uint32_t *d_hist
 
 static_assert(HGAP_RANGE_BYTES % (1024u * 64u) == 0, "ranges hold whole tiles of every shape");
 
-// Workgroup size of the default encoder shape for a codebook: the configured one, halved (down
-// to 256) while two workgroups with input staging would not fit a CU's LDS.  The LDS image is
-// sized for max_len bits per input byte, so long codes (16-bit tails) would otherwise leave one
-// workgroup per CU; on 16-bit codes at 512 MiB, 256 x 64 B tiles encode 27% faster (chained)
-// and 13% faster (per-tile look-back) than 512 x 64 B.  Inputs under HGAP_CHAINED_MIN_BYTES
-// keep the configured size: they have too few tiles for occupancy to matter, and smaller tiles
-// only lengthen the look-back chain (16 MiB: 0.040 against 0.029 ms).
-static uint32_t default_enc_threads(const hgap_codebook *cb, const hgap_launch_config &c,
-                                    uint64_t n) {
-    constexpr size_t kStaticLds = 1104;  // s_tab, s_wave, s_excl, s_tile of a 1024-lane group
-    if (n < HGAP_CHAINED_MIN_BYTES) return c.enc_threads;
-    uint32_t max_len = 1;
-    for (int b = 0; cb && b < 256; b++)
-        max_len = std::max(max_len, std::min<uint32_t>(cb->length[b], HGAP_MAX_CODE_LEN));
-    uint32_t threads = c.enc_threads;
-    while (threads > 256) {
-        const uint32_t tile = threads * c.enc_bytes_per_thread;
-        const size_t lds =
-            4ull * enc_stage_word(enc_out_words(tile, max_len), enc_gap_words(tile, max_len)) +
-            tile + kStaticLds;
-        if (2 * lds <= kMaxLds) break;
-        threads /= 2;
-    }
-    return threads;
-}
-
 // hgap_encode_cfg, and with d_range_hist the chained encoder
 static hgap_status encode_common(const void *d_in, uint64_t n, const hgap_codebook *cb,
                                  const hgap_sizes *sizes, const uint32_t *d_range_hist,
@@ -2273,8 +2196,10 @@ hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook
                            stream);
     hgap_launch_config c;
     hgap_default_config(&c);
+    c.enc_threads = kDefaultChainedThreads;
+    c.enc_bytes_per_thread = kDefaultChainedBytesPerThread;
     hgap_status st = HGAP_ERR_UNSUPPORTED;
-    for (uint32_t threads = default_enc_threads(cb, c, n); threads >= 256; threads /= 2) {
+    for (uint32_t threads = c.enc_threads; threads >= 256; threads /= 2) {
         c.enc_threads = threads;
         st = encode_common(d_in, n, cb, sizes, d_range_hist, d_payload, d_gap, d_workspace,
                            workspace_bytes, &c, stream);
@@ -2283,15 +2208,14 @@ hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook
     return st;
 }
 
-// The default shape for the codebook (smaller workgroups for long codes); smaller still when
-// its LDS image does not fit at all.
+// Default shape first; when its LDS image does not fit (long codes), smaller tiles.
 hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
                         const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream) {
     hgap_launch_config c;
     hgap_default_config(&c);
     hgap_status st = HGAP_ERR_UNSUPPORTED;
-    for (uint32_t threads = default_enc_threads(cb, c, n); threads >= 256; threads /= 2) {
+    for (uint32_t threads = c.enc_threads; threads >= 256; threads /= 2) {
         c.enc_threads = threads;
         st = hgap_encode_cfg(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
                              &c, stream);

@@ -137,9 +137,7 @@ size_t hgap_encode_workspace_bytes(uint64_t n);
 /* Writes payload[W] and gap[ceil(G/8)] for d_in[0..n) with codebook `cb`; sizes must come from
  * hgap_encode_sizes for the same input.  Replaces encode() (encoder/src/encoder.cu:382-457):
  * no host copies, no memset of the payload, the caller keeps ownership of d_in.  Launch shape:
- * hgap_default_config's; from HGAP_CHAINED_MIN_BYTES of input, the workgroup is halved while two
- * workgroups would not fit a CU's LDS for this codebook's longest code (and at any size, smaller
- * again when one does not fit). */
+ * hgap_default_config's, with smaller workgroups when its LDS image does not fit. */
 hgap_status hgap_encode(const void *d_in, uint64_t n, const hgap_codebook *cb,
                         const hgap_sizes *sizes, uint32_t *d_payload, uint32_t *d_gap,
                         void *d_workspace, size_t workspace_bytes, hgap_stream_t stream);
@@ -160,9 +158,9 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
  * hgap_histogram_ranges over the same input (other counts give a wrong stream, never an access
  * outside the caller's buffers), derives every range's bit offset from them before encoding,
  * and so needs no tile to wait for another.  Workspace: hgap_encode_workspace_bytes(n).
- * cfg NULL = the default shape, falling back to smaller tiles like hgap_encode; inputs under
- * HGAP_CHAINED_MIN_BYTES then take hgap_encode's per-tile look-back instead (too few ranges
- * to fill the device; the counts go unused).  A non-NULL cfg always runs the chained encoder. */
+ * cfg NULL = the chained encoder's own default shape (256 threads x 32 bytes); inputs under
+ * HGAP_CHAINED_MIN_BYTES then take hgap_encode instead (too few ranges to fill the device; the
+ * counts go unused).  A non-NULL cfg always runs the chained encoder with that shape. */
 #define HGAP_RANGE_BYTES (256u * 1024u)
 #define HGAP_CHAINED_MIN_BYTES (64ull << 20)
 uint32_t hgap_range_count(uint64_t n);
