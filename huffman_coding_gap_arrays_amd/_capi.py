@@ -98,6 +98,12 @@ def default_config():
     return c
 
 
+def default_chained_config():
+    c = LaunchConfig()
+    load().hgap_default_chained_config(ctypes.byref(c))
+    return c
+
+
 class HistJob(ctypes.Structure):
     """hgap_hist_job (one buffer of a batched histogram)."""
 
@@ -162,6 +168,7 @@ PROTOTYPES = {
     "hgap_status_string": (ctypes.c_char_p, [_INT]),
     "hgap_device_cu_count": (_INT, []),
     "hgap_default_config": (None, [ctypes.c_void_p]),
+    "hgap_default_chained_config": (None, [ctypes.c_void_p]),
     "hgap_codebook_build": (_INT, [_P, ctypes.POINTER(Codebook)]),
     "hgap_codebook_from_order": (_INT, [_P, _P, _U32, ctypes.POINTER(Codebook)]),
     "hgap_encode_sizes": (_INT, [ctypes.POINTER(Codebook), _P, ctypes.POINTER(Sizes)]),

@@ -194,6 +194,8 @@ class BatchCodec:
             ei, di = pr.choose_indices(predictor, X, em, dm)
             configs = [pr.ENC_CANDIDATES[j] for j in ei]
             dec_configs = [pr.DEC_CANDIDATES[j] for j in di]
+        # without shapes from the caller, chained encode launches take the chained default
+        self.default_enc_shape = configs is None
         self.configs = [dict(c) for c in (configs or [{}] * B)]
         self.dec_configs = [dict(c) for c in (dec_configs or self.configs)]
         self.enc_cfg = _config_arrays(self.configs, B)
@@ -262,6 +264,10 @@ class BatchCodec:
                 ranges = int((-(-self.ns[idx].astype(np.int64) // codec.RANGE_BYTES)).sum())
                 chained = self.chained or ranges >= self.CHAINED_MIN_RANGES
             if chained:
+                if self.default_enc_shape:
+                    dc = _capi.default_chained_config()
+                    cfg.enc_threads = dc.enc_threads
+                    cfg.enc_bytes_per_thread = dc.enc_bytes_per_thread
                 jobs["range_base"] = self.range_base[idx]
                 total = lib.hgap_batch_plan_encode_ranges(_vp(jobs), len(idx), ctypes.byref(cfg))
                 ws = torch.empty(_up(16, lib.hgap_batch_encode_ranges_workspace_bytes(total)),

@@ -2067,6 +2067,13 @@ void hgap_default_config(hgap_launch_config *cfg) {
     cfg->prefix_bits = 0;
 }
 
+void hgap_default_chained_config(hgap_launch_config *cfg) {
+    if (!cfg) return;
+    hgap_default_config(cfg);
+    cfg->enc_threads = kDefaultChainedThreads;
+    cfg->enc_bytes_per_thread = kDefaultChainedBytesPerThread;
+}
+
 hgap_status hgap_histogram(const void *d_in, uint64_t n, uint32_t *d_hist,
                            hgap_stream_t stream) {
     if (!d_hist || (n > 0 && !d_in)) return HGAP_ERR_INVALID_ARG;
@@ -2195,9 +2202,7 @@ hgap_status hgap_encode_ranges(const void *d_in, uint64_t n, const hgap_codebook
         return hgap_encode(d_in, n, cb, sizes, d_payload, d_gap, d_workspace, workspace_bytes,
                            stream);
     hgap_launch_config c;
-    hgap_default_config(&c);
-    c.enc_threads = kDefaultChainedThreads;
-    c.enc_bytes_per_thread = kDefaultChainedBytesPerThread;
+    hgap_default_chained_config(&c);
     hgap_status st = HGAP_ERR_UNSUPPORTED;
     for (uint32_t threads = c.enc_threads; threads >= 256; threads /= 2) {
         c.enc_threads = threads;

@@ -335,6 +335,8 @@ class _Batch:
                     max_len = max(max_len, ml)
                 ranges = int((-(-jobs["n"].astype(np.int64) // codec.RANGE_BYTES)).sum())
                 if ranges >= s.chained_min_ranges:
+                    if s.default_enc_shape:
+                        cfg = _capi.default_chained_config()
                     total = lib.hgap_batch_plan_encode_ranges(_vp(jobs), len(members),
                                                               ctypes.byref(cfg))
                     wsb = lib.hgap_batch_encode_ranges_workspace_bytes(total)
@@ -393,6 +395,8 @@ class Server:
         self.predictor = predictor
         self.codebook_threads = codebook_threads
         self.config = LaunchConfig.from_dict(config or {})
+        # without shapes from the caller, chained encode launches take the chained default
+        self.default_enc_shape = config is None and predictor is None
         self.reserved = 0
         self.peak_reserved = 0
         self.batches = 0

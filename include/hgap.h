@@ -89,6 +89,10 @@ typedef struct hgap_launch_config {
 
 const char *hgap_status_string(int status);
 void hgap_default_config(hgap_launch_config *cfg);
+/* hgap_default_config with the chained encoder's default shape (hgap_encode_ranges,
+ * hgap_batch_encode_ranges); the per-tile look-back encoder's default stays
+ * hgap_default_config's.  No counterpart in the reference (its launch shape is fixed). */
+void hgap_default_chained_config(hgap_launch_config *cfg);
 /* Device (ordinal) properties used by the launchers; -1 when no usable device. */
 int hgap_device_cu_count(void);
 
