@@ -588,15 +588,24 @@ __device__ __forceinline__ void enc_tile(const EncBuffer &buf, uint32_t tile,
     __syncthreads();
 }
 
+// Waves per SIMD enc_kernel asks the compiler for.  Shapes of up to 512 threads x 32 bytes fit
+// in 80 VGPRs (512 x 32 with a 12-byte spill), so six waves share a SIMD: the chained 256 x 32
+// encoder ran 8% faster than at its natural 88 VGPRs (five waves), 512 x 32 16% faster.
+// 64-byte shapes would spill heavily and keep the compiler's choice.  (The batched encoders
+// already fit in 80 VGPRs at 256 x 32.)
+template <int THREADS, int K>
+constexpr int enc_waves_per_eu() {
+    return THREADS <= 512 && K <= 32 ? 6 : 1;
+}
+
 // The next tile's input is loaded into registers one tile ahead (claimed right after the
 // current tile's look-back).  CHAINED: the counter hands out ranges of `tiles_per_range` tiles
 // (buf.status has one word per range, prepared by enc_range_prep_kernel); otherwise single
 // tiles.
 template <int THREADS, int K, bool CHAINED>
-__global__ __launch_bounds__(THREADS) void enc_kernel(EncBuffer buf, EncTable table,
-                                                      uint32_t *__restrict__ counter,
-                                                      uint32_t out_words, uint32_t wpl,
-                                                      uint32_t tiles_per_range) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(enc_waves_per_eu<THREADS, K>())))
+void enc_kernel(EncBuffer buf, EncTable table, uint32_t *__restrict__ counter,
+                uint32_t out_words, uint32_t wpl, uint32_t tiles_per_range) {
     extern __shared__ __attribute__((aligned(16))) uint32_t s_dyn[];
     __shared__ uint32_t s_tab[256];
     __shared__ uint32_t s_wave[THREADS / 64];
