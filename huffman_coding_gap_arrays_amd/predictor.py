@@ -1,10 +1,11 @@
 """Learned launch-shape predictor.
 
-A tiny MLP maps five features of a buffer -- histogram entropy, size, longest and mean code
+Tiny MLPs map five features of a buffer -- histogram entropy, size, longest and mean code
 length, symbol count -- to the predicted relative run time of every candidate launch shape of
 the encoder (27: workgroup size x bytes per thread x look-back window) and of the decoder (45:
-workgroup size x look-back window x first-level table bits).  The shape with the lowest
-prediction that the device can hold (hgap_config_supported) is used.
+workgroup size x look-back window x first-level table bits).  The prediction is the mean of an
+ensemble of such networks trained from different seeds.  The shape with the lowest prediction
+that the device can hold (hgap_config_supported) is used.
 
 Training data are the exhaustive autotuner's sweeps (tuner.py): for every buffer of a sweep, the
 median time of every candidate.  The regression target of candidate c is log2(t_c / t_best)
@@ -235,8 +236,15 @@ class Network:
         return self.params.cpu().numpy().astype(np.float32)
 
 
+# Networks per predictor, trained from different seeds; their predicted costs are averaged.
+# With 384 training buffers one network's held-out regret varies a lot with its seed (the
+# learned excess over the exhaustive optimum ranged from 0.41 to 0.55 of the default shapes'
+# excess over three seeds); the average of four ranged from 0.34 to 0.46 over two sweeps.
+ENSEMBLE = 4
+
+
 class Predictor:
-    """Feature normalisation + the network + the candidate lists."""
+    """Feature normalisation + the networks + the candidate lists."""
 
     def __init__(self, params, mean, std, lo, hi, hidden=HIDDEN, device="cuda"):
         self.mean = np.asarray(mean, np.float64)
@@ -245,28 +253,36 @@ class Predictor:
         self.hi = np.asarray(hi, np.float64)
         self.hidden = int(hidden)
         self.device = device
-        self.net = Network(len(FEATURE_NAMES), self.hidden, N_OUT, device, params=params)
+        # params: [count] for one network or [k, count] for an ensemble
+        self.nets = [Network(len(FEATURE_NAMES), self.hidden, N_OUT, device, params=p)
+                     for p in np.atleast_2d(np.asarray(params, np.float32))]
 
     @classmethod
     def train(cls, X, enc_ms, dec_ms, steps=3000, lr=3e-3, hidden=HIDDEN, seed=0,
-              device="cuda", log_every=0):
-        """Fits a predictor to sweep rows X [N, 5] with candidate times; returns
-        (predictor, loss history)."""
+              device="cuda", log_every=0, ensemble=ENSEMBLE):
+        """Fits a predictor to sweep rows X [N, 5] with candidate times: `ensemble` networks
+        from seeds seed, seed + 1, ...; returns (predictor, loss history of the first)."""
         import torch
 
         X = np.asarray(X, np.float64)
         mean, std = X.mean(0), X.std(0)
         std = np.where(std > 1e-9, std, 1.0)
         T, M = targets(enc_ms, dec_ms)
-        net = Network(len(FEATURE_NAMES), hidden, N_OUT, device, seed=seed)
         dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(device)  # noqa
-        loss = net.fit(dev((X - mean) / std), dev(T), dev(M), steps, lr, log_every)
-        p = cls(net.host_params(), mean, std, X.min(0), X.max(0), hidden, device)
+        x, t, m = dev((X - mean) / std), dev(T), dev(M)
+        params, loss = [], None
+        for k in range(max(1, int(ensemble))):
+            net = Network(len(FEATURE_NAMES), hidden, N_OUT, device, seed=seed + k)
+            hist = net.fit(x, t, m, steps, lr, log_every if k == 0 else 0)
+            loss = hist if k == 0 else loss
+            params.append(net.host_params())
+        p = cls(np.stack(params), mean, std, X.min(0), X.max(0), hidden, device)
         return p, loss
 
     def save(self, path):
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-        np.savez(path, params=self.net.host_params(), mean=self.mean, std=self.std, lo=self.lo,
+        params = np.stack([net.host_params() for net in self.nets])
+        np.savez(path, params=params, mean=self.mean, std=self.std, lo=self.lo,
                  hi=self.hi, hidden=np.int64(self.hidden),
                  enc_candidates=json.dumps(ENC_CANDIDATES),
                  dec_candidates=json.dumps(DEC_CANDIDATES),
@@ -281,15 +297,16 @@ class Predictor:
         return cls(z["params"], z["mean"], z["std"], z["lo"], z["hi"], int(z["hidden"]), device)
 
     def costs(self, X):
-        """Predicted log2 relative cost [N, N_OUT] (host float64) of every candidate."""
+        """Predicted log2 relative cost [N, N_OUT] (host float64) of every candidate: the mean
+        over the networks."""
         import torch
 
         X = np.atleast_2d(np.asarray(X, np.float64))
         if len(X) == 0:
             return np.zeros((0, N_OUT))
         x = torch.from_numpy(((X - self.mean) / self.std).astype(np.float32)).to(self.device)
-        _, y = self.net.forward(x)
-        return y.cpu().numpy().astype(np.float64)
+        ys = [net.forward(x)[1].cpu().numpy().astype(np.float64) for net in self.nets]
+        return np.mean(ys, axis=0)
 
     def in_range(self, X, margin=0.05):
         X = np.atleast_2d(np.asarray(X, np.float64))
