@@ -162,9 +162,11 @@ hgap_status hgap_encode_cfg(const void *d_in, uint64_t n, const hgap_codebook *c
  * hgap_histogram_ranges over the same input (other counts give a wrong stream, never an access
  * outside the caller's buffers), derives every range's bit offset from them before encoding,
  * and so needs no tile to wait for another.  Workspace: hgap_encode_workspace_bytes(n).
- * cfg NULL = the chained encoder's own default shape (256 threads x 32 bytes); inputs under
- * HGAP_CHAINED_MIN_BYTES then take hgap_encode instead (too few ranges to fill the device; the
- * counts go unused).  A non-NULL cfg always runs the chained encoder with that shape. */
+ * cfg NULL = the chained encoder's own default shape (256 threads x 32 bytes), or 512 x 32
+ * when its resident workgroups divide the ranges clearly more evenly (a workgroup encodes
+ * whole ranges); inputs under HGAP_CHAINED_MIN_BYTES then take hgap_encode instead (too few
+ * ranges to fill the device; the counts go unused).  A non-NULL cfg always runs the chained
+ * encoder with that shape. */
 #define HGAP_RANGE_BYTES (256u * 1024u)
 #define HGAP_CHAINED_MIN_BYTES (64ull << 20)
 uint32_t hgap_range_count(uint64_t n);

@@ -312,7 +312,13 @@ def test_full_size_configs(codec, orc, golden_codebooks, name):
     enc2 = codec.encode(d_dev, cb=enc.codebook, sizes=enc.sizes, range_hist=rh, config={})
     assert torch.equal(enc2.payload[:W], enc.payload[:W]), "chained payload"
     assert torch.equal(enc2.gap[:GW], enc.gap[:GW]), "chained gap"
-    del enc2, rh
+    del enc2
+    # without a shape: the library's own choice (256 x 32, or 512 x 32 where its workgroups
+    # divide the ranges more evenly; look-back below 64 MiB)
+    enc3 = codec.encode(d_dev, cb=enc.codebook, sizes=enc.sizes, range_hist=rh)
+    assert torch.equal(enc3.payload[:W], enc.payload[:W]), "default chained payload"
+    assert torch.equal(enc3.gap[:GW], enc.gap[:GW]), "default chained gap"
+    del enc3, rh
     say("  oracle encode", name)
     d = d_dev.cpu().numpy()
     del out
